@@ -1,0 +1,117 @@
+/* vsg: the C ABI of the MI355X exact k-NN engine (libvsg_hip.so, built for gfx950).
+ *
+ * Plain pointers and sizes only. Every entry point returns a status code; on failure the
+ * calling thread's message is in vsg_last_error(). This is the boundary the reference's
+ * GpuBruteForceIndex (include/vecsearch/gpu_brute_force.h:26-50, src/gpu_brute_force.cu:
+ * 334-488) is rebuilt behind: the C++ class in include/vecsearch/gpu_brute_force.h and the
+ * pybind11 module call only these functions.
+ *
+ * Semantics shared by every search entry point (gpu_brute_force.h:40-43, brute_force.cpp:
+ * 152-168, distance.h:1-3,71-73):
+ *   - distances, not similarities: squared L2, or 1 - <q, x> for inner product / cosine;
+ *   - nq x k outputs, row-major, nearest first, ties broken by the lower id;
+ *   - a missing result (k > number of candidate rows) is id VSG_NONE with distance +inf;
+ *   - every returned distance is bit-identical to the reference CPU index's FP32 value.
+ */
+#ifndef VSG_H
+#define VSG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vsg_index vsg_index;
+
+/* status codes; the C++ class maps them onto the reference's exception types
+ * (gpu_brute_force.cu:11-16, 297, 387, 408) */
+#define VSG_OK 0
+#define VSG_ERR_INVALID_ARGUMENT 1 /* std::invalid_argument -> Python ValueError */
+#define VSG_ERR_LENGTH 2           /* std::length_error */
+#define VSG_ERR_OUT_OF_MEMORY 3    /* std::runtime_error (device allocation failed) */
+#define VSG_ERR_DEVICE 4           /* std::runtime_error (HIP runtime error) */
+#define VSG_ERR_NO_DEVICE 5        /* std::runtime_error (no GPU visible) */
+
+/* Metric (distance.h:15) */
+#define VSG_METRIC_L2 0
+#define VSG_METRIC_IP 1
+
+/* Search method (gpu_brute_force.h:16-21). AUTO picks by batch size; NAIVE, SKINNY and TILED
+ * force the one-query streaming scan, the multi-query streaming scan and the MFMA shortlist
+ * scan respectively. All of them return the same ids and distances. */
+#define VSG_METHOD_AUTO 0
+#define VSG_METHOD_NAIVE 1
+#define VSG_METHOD_SKINNY 2
+#define VSG_METHOD_TILED 3
+
+#define VSG_NONE 0xFFFFFFFFu /* gpu_brute_force.h:28 kNone */
+#define VSG_MAX_K 128        /* gpu_brute_force.h:29 kMaxK */
+
+/* Thread-local text of the last failure on this thread ("" if none). */
+const char* vsg_last_error(void);
+
+/* Number of visible HIP devices (0 without a driver); replaces cuda_available()
+ * (gpu_brute_force.cu:334-341). */
+int vsg_device_count(int* count);
+
+/* GpuBruteForceIndex(dim, metric, device) (gpu_brute_force.cu:386-396). dim == 0 -> INVALID. */
+int vsg_create(size_t dim, int metric, int device, vsg_index** out);
+void vsg_destroy(vsg_index* h);
+
+int vsg_size(const vsg_index* h, size_t* n);
+int vsg_dim(const vsg_index* h, size_t* dim);
+int vsg_metric(const vsg_index* h, int* metric);
+
+/* Make room for n rows without copying on later adds. */
+int vsg_reserve(vsg_index* h, size_t n);
+
+/* add(data, n) (gpu_brute_force.cu:404-423): host rows, n x dim, appended with ids size()..;
+ * size() + n >= 2^32 - 1 -> LENGTH. */
+int vsg_add(vsg_index* h, const float* data, size_t n);
+
+/* search(queries, nq, k, out_ids, out_dists, method) (gpu_brute_force.cu:425-488) with host
+ * buffers, including the query upload and the result download. k > VSG_MAX_K -> INVALID.
+ * mask (nullable) has size() entries; a nonzero entry allows that row for every query, the
+ * convention of SearchFilter.allowed (hnsw.h:29-33) and of the `filter` argument of the
+ * Python bindings (bindings.cpp:21-36). With a mask the result is the exact scan over the
+ * allowed rows (hnsw.cpp:336-354); no allowed rows gives an all-missing result
+ * (hnsw.cpp:326). */
+int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int method,
+               const uint8_t* mask, uint32_t* out_ids, float* out_dists);
+
+/* The same search on device-resident buffers (queries nq x dim fp32, out_ids / out_dists
+ * nq x k), enqueued on `stream` (a hipStream_t; NULL = the index's own stream). Returns after
+ * enqueueing; the outputs are valid once the stream reaches that point. */
+int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
+                      const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream);
+
+/* Batched filtered exact scan with a per-query k and per-query allowed-id lists: query i may
+ * return only rows allow_ids[range[2i] .. range[2i+1]) (strictly increasing, < size()), and
+ * returns ks[i] <= kmax results. Ranges may coincide: queries with identical ranges share
+ * their row reads. n_ids = len(allow_ids) (every range must lie inside it). Outputs are
+ * nq x kmax with columns ks[i].. padded. Host buffers. This is HNSWIndex::search's
+ * exact_below path (hnsw.cpp:322-354) for many queries with different filters in one call. */
+int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
+                     size_t kmax, const uint64_t* range, const uint32_t* allow_ids, size_t n_ids,
+                     uint32_t* out_ids, float* out_dists);
+
+/* Kernel timing for measurement: while enabled, every search records HIP events around its
+ * scan launches and its merge launches on the stream they run on. vsg_kernel_times waits for
+ * the recorded events, returns the accumulated milliseconds and launch counts since the last
+ * call, and resets them. */
+int vsg_kernel_timing(vsg_index* h, int enable);
+int vsg_kernel_times(vsg_index* h, double* scan_ms, uint64_t* scan_launches, double* merge_ms,
+                     uint64_t* merge_launches);
+
+/* Wait for all work the index has enqueued on its own stream. */
+int vsg_synchronize(vsg_index* h);
+
+/* Device pointer of the resident base (size() x padded_dim fp32, row-major, zero padded),
+ * for measurement code that needs the exact HBM footprint. */
+int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,190 @@
+"""HIP engine vs the CPU oracle: ids identical and distances bit-identical.
+
+Mirrors the reference's GPU tests (tests/python/test_gpu.py:17-70) with the bar raised from
+"allclose, ids > 99 %" to exact equality, plus the filtered paths (test_bindings.py:76-90,
+tests/cpp/test_filter.cpp:87-121) and the mixture workload of the benchmark at 100k rows.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import vecsearch_amd as vs
+from vecsearch_amd import datagen as dg
+
+pytestmark = pytest.mark.gpu
+METHODS = ["naive", "skinny", "tiled", "auto"]
+
+
+def data(n, d, seed, normalize=False):
+    x = np.random.default_rng(seed).standard_normal((n, d), dtype=np.float32)
+    return vs.normalize(x) if normalize else x
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def as_i64(ids):
+    return np.where(ids == oracle.NONE, -1, ids.astype(np.int64))
+
+
+def assert_same(got, exp):
+    gi, gd = got
+    ei, ed = exp
+    assert gi.shape == ei.shape
+    np.testing.assert_array_equal(gi, as_i64(ei))
+    np.testing.assert_array_equal(bits(gd), bits(ed))
+
+
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("d", [37, 128])
+@pytest.mark.parametrize("nq", [1, 5, 300])
+def test_matches_oracle(metric, method, d, nq):
+    x, q, k = data(3000, d, 1, metric == "ip"), data(nq, d, 2, metric == "ip"), 10
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x[:1000])  # two adds: the device buffer grows and keeps the first part
+    gpu.add(x[1000:])
+    assert len(gpu) == 3000 and gpu.dim == d
+    got = gpu.search(q, k, method=method)
+    assert_same(got, oracle.search(x, q, k, metric, threads=8))
+    assert np.all(np.diff(got[1], axis=1) >= 0)
+
+
+@pytest.mark.parametrize("d", [1, 3, 7, 8, 9, 15, 16, 17, 24, 31, 45, 384, 385, 768])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_every_tail_length(d, metric):
+    x, q = data(700, d, 3, metric == "ip"), data(9, d, 4, metric == "ip")
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    assert_same(gpu.search(q, 12), oracle.search(x, q, 12, metric, threads=8))
+
+
+@pytest.mark.parametrize("k", [1, 16, 17, 64, 65, 100, 128])
+def test_k_buckets(k):
+    x, q = data(20000, 64, 3), data(40, 64, 4)
+    gpu = vs.GpuBruteForceIndex(64)
+    gpu.add(x)
+    assert_same(gpu.search(q, k), oracle.search(x, q, k, "l2", threads=8))
+
+
+def test_many_queries_and_rounds():
+    x, q = data(70000, 16, 5), data(5000, 16, 6)
+    gpu = vs.GpuBruteForceIndex(16)
+    gpu.add(x)
+    assert_same(gpu.search(q, 10), oracle.search(x, q, 10, "l2", threads=8))
+
+
+def test_exact_ties_go_to_lower_id():
+    x = data(4000, 32, 7)
+    x[100:140] = x[5]  # 41 identical rows
+    gpu = vs.GpuBruteForceIndex(32)
+    gpu.add(x)
+    q = x[5:6] + 0.01
+    ids, dists = gpu.search(q, 50)
+    assert_same((ids, dists), oracle.search(x, q, 50, "l2"))
+    assert list(ids[0, :41]) == [5] + list(range(100, 140))
+
+
+def test_missing_results_and_limits():
+    gpu = vs.GpuBruteForceIndex(8)
+    ids, dists = gpu.search(np.zeros((2, 8), np.float32), 5)  # empty index
+    assert (ids == -1).all() and np.isinf(dists).all()
+    gpu.add(data(3, 8, 7))
+    ids, dists = gpu.search(np.zeros(8, np.float32), 5)
+    assert (ids[0, :3] >= 0).all() and (ids[0, 3:] == -1).all() and np.isinf(dists[0, 3:]).all()
+    with pytest.raises(ValueError):
+        gpu.search(np.zeros(8, np.float32), 129)
+    with pytest.raises(ValueError):
+        gpu.search(np.zeros(8, np.float32), 5, method="fast")
+    with pytest.raises(ValueError):
+        gpu.search(np.zeros((2, 9), np.float32), 5)
+    ids, dists = gpu.search(np.zeros((0, 8), np.float32), 5)
+    assert ids.shape == (0, 5)
+
+
+def test_forcecast_float64():
+    x, q = data(500, 24, 8), data(4, 24, 9)
+    gpu = vs.GpuBruteForceIndex(24, "cosine")
+    gpu.add(x.astype(np.float64))
+    assert_same(gpu.search(q.astype(np.float64), 7), oracle.search(x, q, 7, "ip"))
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.3, 0.01, 0.0005, 0.0])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_mask_filter(frac, metric):
+    x, q = data(30000, 64, 10, metric == "ip"), data(17, 64, 11, metric == "ip")
+    mask = np.random.default_rng(12).random(len(x)) < frac
+    gpu = vs.GpuBruteForceIndex(64, metric)
+    gpu.add(x)
+    got = gpu.search(q, 10, filter=mask)
+    assert_same(got, oracle.search(x, q, 10, metric, threads=8, mask=mask))
+    ids = got[0]
+    assert mask[ids[ids >= 0]].all()
+    # test_bindings.py:76-90: equal to brute force over x[mask] mapped through flatnonzero
+    if mask.any():
+        sub = vs.GpuBruteForceIndex(64, metric)
+        sub.add(x[mask])
+        gi, gd = sub.search(q, 10)
+        mapped = np.where(gi >= 0, np.flatnonzero(mask)[np.maximum(gi, 0)], -1)
+        np.testing.assert_array_equal(ids, mapped)
+    with pytest.raises(ValueError):
+        gpu.search(q, 10, filter=np.ones(5, bool))
+
+
+def test_filter_fewer_allowed_than_k():
+    x, q = data(5000, 32, 13), data(3, 32, 14)
+    mask = np.zeros(len(x), bool)
+    mask[[7, 4000]] = True
+    gpu = vs.GpuBruteForceIndex(32)
+    gpu.add(x)
+    ids, dists = gpu.search(q, 10, filter=mask)
+    assert set(ids[0, :2]) == {7, 4000} and (ids[:, 2:] == -1).all() and np.isinf(dists[:, 2:]).all()
+    assert_same((ids, dists), oracle.search(x, q, 10, "l2", mask=mask))
+
+
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_search_batch_lists(metric):
+    n, d = 40000, 96
+    x = dg.mixture(n, d, seed=20)
+    groups = dg.zipf_groups(n, 50, 1.1, seed=21)
+    off, ids_g = dg.group_csr(groups, 50)
+    q = dg.mixture(300, d, seed=22)
+    ks, qg = dg.query_filters(300, 50, seed=23, ks=(1, 10, 100))
+    ranges = dg.query_ranges(qg, off)
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    got = gpu.search_batch(q, ks, ranges, ids_g)
+    exp = oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=100)
+    assert_same(got, exp)
+
+
+def test_search_batch_edge_cases():
+    x = data(3000, 20, 30)
+    gpu = vs.GpuBruteForceIndex(20)
+    gpu.add(x)
+    q = data(5, 20, 31)
+    allow = np.array([3, 9, 10, 2999], np.uint32)
+    ranges = np.array([[0, 4], [0, 0], [1, 3], [0, 4], [3, 4]], np.uint64)
+    ks = np.array([10, 5, 1, 0, 2], np.uint32)
+    got = gpu.search_batch(q, ks, ranges, allow)
+    assert_same(got, oracle.search_lists(x, q, ks, ranges, allow, "l2", kmax=10))
+    assert (got[0][1] == -1).all() and (got[0][3] == -1).all()
+    with pytest.raises(ValueError):  # ids must be increasing
+        gpu.search_batch(q[:1], np.array([3], np.uint32), np.array([[0, 2]], np.uint64),
+                         np.array([9, 3], np.uint32))
+    with pytest.raises(ValueError):  # id out of range
+        gpu.search_batch(q[:1], np.array([3], np.uint32), np.array([[0, 1]], np.uint64),
+                         np.array([3000], np.uint32))
+    with pytest.raises(ValueError):  # range past the id array
+        gpu.search_batch(q[:1], np.array([3], np.uint32), np.array([[0, 5]], np.uint64), allow)
+
+
+@pytest.mark.parametrize("batch", [1, 8, 100])
+def test_mixture_100k_384(batch):
+    # the CPU-runnable benchmark shape: 100,000 x 384 unit rows, cosine, top-10
+    x = dg.mixture(100_000, 384, seed=0)
+    q = dg.mixture(batch, 384, seed=1)
+    gpu = vs.GpuBruteForceIndex(384, "cosine")
+    gpu.add(x)
+    assert_same(gpu.search(q, 10), oracle.search(x, q, 10, "ip", threads=8))

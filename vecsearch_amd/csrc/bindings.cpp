@@ -1,0 +1,202 @@
+// pybind11 module vecsearch_amd._vecsearch: the reference's GPU binding surface
+// (python/bindings.cpp:18-56, 151-197) over the MI355X engine.
+//
+// Kept from the reference: FloatArray forcecast (float64 -> float32 copy, :18), 1-D queries
+// become (1, d) (:174), check_2d -> ValueError (:44-47), mask parsing as_mask (:28-36),
+// parse_metric (:38-42), ids as int64 with -1 for missing (:49-56), the GIL released around
+// add/search (:161, :179-182), method names auto/naive/skinny/tiled (:168-173).
+// Added: search(..., filter=None) with the HNSW binding's mask convention (:84-91), and
+// search_batch for per-query k and per-query allowed-id lists.
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include "vecsearch/gpu_brute_force.h"
+#include "vsg.h"
+
+namespace py = pybind11;
+using namespace vecsearch;
+using FloatArray = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using MaskArray = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+struct Mask {
+  MaskArray arr;
+  const uint8_t* ptr = nullptr;
+};
+
+Mask as_mask(const py::object& filter, size_t n) {
+  Mask m;
+  if (filter.is_none()) return m;
+  m.arr = MaskArray::ensure(filter);
+  if (!m.arr || m.arr.ndim() != 1 || size_t(m.arr.shape(0)) != n)
+    throw std::invalid_argument("filter must be a 1-D bool array of length " + std::to_string(n));
+  m.ptr = m.arr.data();
+  return m;
+}
+
+Metric parse_metric(const std::string& s) {
+  if (s == "l2") return Metric::L2;
+  if (s == "ip" || s == "cosine") return Metric::InnerProduct;
+  throw std::invalid_argument(
+      "metric must be 'l2', 'ip' or 'cosine' (cosine expects normalized vectors)");
+}
+
+GpuMethod parse_method(const std::string& method) {
+  if (method == "auto") return GpuMethod::Auto;
+  if (method == "naive") return GpuMethod::Naive;
+  if (method == "skinny") return GpuMethod::Skinny;
+  if (method == "tiled") return GpuMethod::Tiled;
+  throw std::invalid_argument("method must be 'auto', 'naive', 'skinny' or 'tiled'");
+}
+
+void check_2d(const FloatArray& a, size_t dim) {
+  if (a.ndim() != 2 || size_t(a.shape(1)) != dim)
+    throw std::invalid_argument("expected array of shape (n, " + std::to_string(dim) + ")");
+}
+
+py::array_t<int64_t> ids_to_int64(const py::array_t<uint32_t>& ids) {
+  py::array_t<int64_t> out({ids.shape(0), ids.shape(1)});
+  const uint32_t* src = ids.data();
+  int64_t* dst = out.mutable_data();
+  for (py::ssize_t i = 0; i < ids.size(); ++i)
+    dst[i] = src[i] == 0xFFFFFFFFu ? -1 : int64_t(src[i]);
+  return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_vecsearch, m) {
+  m.doc() = "Exact k-NN on an AMD Instinct MI355X (HIP, gfx950)";
+
+  py::enum_<Metric>(m, "Metric").value("L2", Metric::L2).value("InnerProduct", Metric::InnerProduct);
+
+  py::class_<GpuBruteForceIndex>(m, "GpuBruteForceIndex")
+      .def(py::init([](size_t dim, const std::string& metric, int device) {
+             return std::make_unique<GpuBruteForceIndex>(dim, parse_metric(metric), device);
+           }),
+           py::arg("dim"), py::arg("metric") = "l2", py::arg("device") = 0)
+      .def(
+          "add",
+          [](GpuBruteForceIndex& self, FloatArray x) {
+            check_2d(x, self.dim());
+            py::gil_scoped_release nogil;
+            self.add(x.data(), x.shape(0));
+          },
+          py::arg("x"))
+      .def(
+          "search",
+          [](GpuBruteForceIndex& self, FloatArray q, size_t k, const py::object& filter,
+             const std::string& method) {
+            const GpuMethod gm = parse_method(method);
+            if (q.ndim() == 1) q = q.reshape({py::ssize_t(1), q.shape(0)});
+            check_2d(q, self.dim());
+            const size_t nq = q.shape(0);
+            Mask mask = as_mask(filter, self.size());
+            py::array_t<uint32_t> ids({nq, k});
+            py::array_t<float> dists({nq, k});
+            {
+              py::gil_scoped_release nogil;
+              if (mask.ptr)
+                self.search(q.data(), nq, k, ids.mutable_data(), dists.mutable_data(), mask.ptr, gm);
+              else
+                self.search(q.data(), nq, k, ids.mutable_data(), dists.mutable_data(), gm);
+            }
+            return py::make_tuple(ids_to_int64(ids), dists);
+          },
+          py::arg("queries"), py::arg("k") = 10, py::arg("filter") = py::none(), py::kw_only(),
+          py::arg("method") = "auto",
+          "Exact top-k on the GPU, k <= 128. Returns (ids[int64, nq x k], distances[float32, nq x k]).\n"
+          "filter: optional bool/uint8 mask over ids (len(self) entries); only allowed ids are returned.")
+      .def(
+          "search_batch",
+          [](GpuBruteForceIndex& self, FloatArray q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,
+             py::array_t<uint64_t, py::array::c_style | py::array::forcecast> ranges,
+             py::array_t<uint32_t, py::array::c_style | py::array::forcecast> allow_ids, py::object kmax_obj) {
+            if (q.ndim() == 1) q = q.reshape({py::ssize_t(1), q.shape(0)});
+            check_2d(q, self.dim());
+            const size_t nq = q.shape(0);
+            if (ks.ndim() != 1 || size_t(ks.shape(0)) != nq)
+              throw std::invalid_argument("ks must be a 1-D array with one k per query");
+            if (ranges.ndim() != 2 || size_t(ranges.shape(0)) != nq || ranges.shape(1) != 2)
+              throw std::invalid_argument("ranges must have shape (nq, 2): [begin, end) into allow_ids");
+            if (allow_ids.ndim() != 1)
+              throw std::invalid_argument("allow_ids must be a 1-D array");
+            const size_t n_ids = size_t(allow_ids.shape(0));
+            size_t kmax = 0;
+            for (size_t i = 0; i < nq; ++i) kmax = std::max<size_t>(kmax, ks.data()[i]);
+            if (!kmax_obj.is_none()) kmax = std::max(kmax, kmax_obj.cast<size_t>());
+            py::array_t<uint32_t> ids({nq, kmax});
+            py::array_t<float> dists({nq, kmax});
+            {
+              py::gil_scoped_release nogil;
+              self.search_lists(q.data(), nq, ks.data(), kmax, ranges.data(), allow_ids.data(),
+                                n_ids, ids.mutable_data(), dists.mutable_data());
+            }
+            return py::make_tuple(ids_to_int64(ids), dists);
+          },
+          py::arg("queries"), py::arg("ks"), py::arg("ranges"), py::arg("allow_ids"),
+          py::arg("kmax") = py::none(),
+          "Batched filtered exact scan. Query i returns ks[i] results among ids\n"
+          "allow_ids[ranges[i,0]:ranges[i,1]] (strictly increasing); queries with identical\n"
+          "ranges share row reads. Returns (ids[int64, nq x kmax], distances[float32, nq x kmax]);\n"
+          "columns past ks[i] are -1 / inf.")
+      .def(
+          "search_device",
+          [](GpuBruteForceIndex& self, uintptr_t d_queries, size_t nq, size_t k, uintptr_t d_ids,
+             uintptr_t d_dists, uintptr_t stream, const std::string& method, uintptr_t d_mask) {
+            const GpuMethod gm = parse_method(method);
+            int rc;
+            {
+              py::gil_scoped_release nogil;
+              rc = vsg_search_device(self.handle(), reinterpret_cast<const float*>(d_queries), nq, k,
+                                     static_cast<int>(gm), reinterpret_cast<const uint8_t*>(d_mask),
+                                     reinterpret_cast<uint32_t*>(d_ids),
+                                     reinterpret_cast<float*>(d_dists), reinterpret_cast<void*>(stream));
+            }
+            if (rc == VSG_ERR_INVALID_ARGUMENT) throw std::invalid_argument(vsg_last_error());
+            if (rc != VSG_OK) throw std::runtime_error(vsg_last_error());
+          },
+          py::arg("d_queries"), py::arg("nq"), py::arg("k"), py::arg("d_ids"), py::arg("d_dists"),
+          py::arg("stream") = 0, py::arg("method") = "auto", py::arg("d_mask") = 0,
+          "Enqueue a search on device buffers (raw device pointers: queries nq x dim float32,\n"
+          "ids nq x k uint32, dists nq x k float32) on `stream` (0 = the index's own stream).")
+      .def(
+          "kernel_timing",
+          [](GpuBruteForceIndex& self, bool enable) {
+            if (vsg_kernel_timing(self.handle(), enable ? 1 : 0) != VSG_OK)
+              throw std::runtime_error(vsg_last_error());
+          },
+          py::arg("enable"))
+      .def("kernel_times",
+           [](GpuBruteForceIndex& self) {
+             double s = 0, m = 0;
+             uint64_t sl = 0, ml = 0;
+             if (vsg_kernel_times(self.handle(), &s, &sl, &m, &ml) != VSG_OK)
+               throw std::runtime_error(vsg_last_error());
+             py::dict d;
+             d["scan_ms"] = s;
+             d["scan_launches"] = sl;
+             d["merge_ms"] = m;
+             d["merge_launches"] = ml;
+             return d;
+           },
+           "Accumulated kernel milliseconds and launch counts since the last call (HIP events).")
+      .def("synchronize", [](GpuBruteForceIndex& self) {
+        py::gil_scoped_release nogil;
+        if (vsg_synchronize(self.handle()) != VSG_OK) throw std::runtime_error(vsg_last_error());
+      })
+      .def_property_readonly("dim", &GpuBruteForceIndex::dim)
+      .def_property_readonly("metric", [](const GpuBruteForceIndex& self) {
+        return self.metric() == Metric::L2 ? "l2" : "ip";
+      })
+      .def("__len__", &GpuBruteForceIndex::size);
+
+  m.def("cuda_available", [] { return cuda_available(); },
+        "True if a GPU is visible (API-compatible name; the runtime is HIP).");
+  m.def("hip_available", [] { return hip_available(); }, "True if a HIP GPU is visible.");
+}

@@ -1,0 +1,43 @@
+// Internal interfaces between the C-ABI layer (vsg_api.hip) and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace vsg {
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = 64 * kWavesPerBlock;
+constexpr int kMergeHeads = 16;  // lists per merge thread: a query merges <= 256 * 16 lists
+constexpr uint32_t kMaxListsPerQuery = 256 * kMergeHeads;
+constexpr uint32_t kNoneId = 0xFFFFFFFFu;
+
+// vsg_scan.hip
+hipError_t launch_scan(int qb, bool l2, uint32_t k, uint32_t n_blocks, hipStream_t st,
+                       const float* base, uint32_t n, uint32_t dp, uint32_t d, const float* q,
+                       uint32_t q0, uint32_t nqp, uint32_t rpw, const uint8_t* mask,
+                       uint64_t* part);
+hipError_t launch_merge(uint32_t n_out, const uint64_t* part, const uint64_t* list_off, uint32_t W,
+                        uint32_t L, const uint32_t* ks, uint32_t k, const uint32_t* qmap,
+                        uint32_t* ids, float* dists, uint32_t ldo, hipStream_t st);
+hipError_t launch_fill_missing(hipStream_t st, uint32_t* ids, float* dists, size_t count);
+
+// vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
+struct ListsWorkspace {
+  void* dev[8] = {};      // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out)
+  size_t dev_cap[8] = {};
+  void* host[3] = {};     // pinned staging (queries, ids out, dists out)
+  size_t host_cap[3] = {};
+  uint32_t* counter = nullptr;
+  void release();
+};
+
+int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t dim, size_t dp,
+                      int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
+                      const uint32_t* ks, size_t kmax, const uint64_t* range,
+                      const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
+                      float* out_dists, std::string& err);
+
+}  // namespace vsg

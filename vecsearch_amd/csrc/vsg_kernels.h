@@ -1,0 +1,207 @@
+// Device helpers shared by the vsg kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vsg {
+
+constexpr uint64_t kPadKey = ~0ull;  // sorts after every real (dist, id); decodes to (inf, kNone)
+
+// Monotone map of a float onto uint32 (total order; -0 and +0 are canonicalised to +0 so the
+// order matches IEEE `<` on every non-NaN distance).
+__device__ __forceinline__ uint32_t ord_bits(float f) {
+  uint32_t u = __float_as_uint(f == 0.f ? 0.f : f);
+  return u ^ ((u & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u);
+}
+__device__ __forceinline__ float unord_bits(uint32_t o) {
+  uint32_t u = o ^ ((o & 0x80000000u) ? 0x80000000u : 0xFFFFFFFFu);
+  return __uint_as_float(u);
+}
+// A candidate as one 64-bit key: (dist, id) lexicographic order == unsigned order of the key.
+// This is the tie rule of the reference merge (brute_force.cpp:159-161).
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
+  return (uint64_t(ord_bits(d)) << 32) | id;
+}
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return uint32_t(k); }
+__device__ __forceinline__ float key_dist(uint64_t k) {
+  return k == kPadKey ? __builtin_inff() : unord_bits(uint32_t(k >> 32));
+}
+
+// quad_perm DPP controls: lane t of each group of four reads lane perm[t].
+constexpr int kDppXor2 = 0x4E;  // [2,3,0,1]
+constexpr int kDppXor1 = 0xB1;  // [1,0,3,2]
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_up1_64(uint64_t v, int lane) {
+  // value of lane-1 (lane 0 gets its own value; callers overwrite it)
+  const int src = (lane == 0 ? 0 : lane - 1) << 2;
+  const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, int(uint32_t(v)));
+  const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, int(uint32_t(v >> 32)));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Sorted k-list of one query, spread over the wave: slot s lives in lane s & 63 of register
+// s >> 6 (KP = 64 or 128 slots). insert() keeps the list sorted ascending and returns the new
+// k-th key (the admission threshold). The key must be smaller than the current k-th key.
+template <int KP>
+struct WaveList {
+  uint64_t e[KP / 64];
+
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int r = 0; r < KP / 64; ++r) e[r] = kPadKey;
+  }
+
+  __device__ __forceinline__ uint64_t insert(uint64_t key, int lane, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int r = 0; r < KP / 64; ++r) pos += __popcll(ballot(e[r] < key));
+    uint64_t prev[KP / 64];
+#pragma unroll
+    for (int r = 0; r < KP / 64; ++r) {
+      prev[r] = shfl_up1_64(e[r], lane);
+      if (lane == 0) prev[r] = r == 0 ? kPadKey : readlane64(e[r - 1], 63);
+    }
+#pragma unroll
+    for (int r = 0; r < KP / 64; ++r) {
+      const int s = r * 64 + lane;
+      e[r] = s < pos ? e[r] : (s == pos ? key : prev[r]);
+    }
+    const int kk = k - 1;
+    uint64_t thr = kPadKey;
+#pragma unroll
+    for (int r = 0; r < KP / 64; ++r)
+      if ((kk >> 6) == r) thr = readlane64(e[r], kk & 63);
+    return thr;
+  }
+};
+
+template <bool L2>
+__device__ __forceinline__ float fma_term(float q, float x, float acc) {
+  if (L2) {
+    const float e = q - x;  // brute_force.cpp:28-31 (q0 = q - x; fma(q0, q0, acc))
+    return __builtin_fmaf(e, e, acc);
+  }
+  return __builtin_fmaf(q, x, acc);  // brute_force.cpp:33-34
+}
+
+// Scalar tail of d % 8 dims after the tree (distance.h:50, 68), as the reference's release
+// build computes it: one block of 4 rounded products added in order, then fused terms.
+template <bool L2>
+__device__ __forceinline__ float scalar_tail(const float* q, const float* __restrict__ x, int i,
+                                             int d, float r) {
+  if (i + 4 <= d) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j, ++i) {
+      const float a = q[i], b = x[i];
+      const float p = L2 ? __fmul_rn(a - b, a - b) : __fmul_rn(a, b);
+      r = __fadd_rn(r, p);
+    }
+  }
+  for (; i < d; ++i) r = fma_term<L2>(q[i], x[i], r);
+  return r;
+}
+
+// Reference-order distances of R rows x QB queries for the calling quad lane t.
+// xr[j]: start of row j (row-major, dp floats, 64-byte aligned); ok[j] false skips the loads.
+// q: QB queries in LDS, query b at q + b * dp (zero beyond d). Every lane of a quad returns the
+// same dist[j][b]; all 64 lanes of the wave must call this together (DPP across the quad).
+//
+// Accumulator layout: lane t keeps the reference's accumulators 4t..4t+3, i.e. dims 4t..4t+3
+// mod 16 (acc0 = accumulators 0..7, acc1 = 8..15 in distance.h:57-61). After the 16-dim steps
+// (and the 8-dim step into acc0), the tree of distance.h:18-27 runs across the quad with DPP:
+// v = acc + acc[lane ^ 2], w = v + v[lane ^ 1], r = (w0 + w1) + (w2 + w3).
+template <int QB, int R, int D16, bool L2>
+__device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bool (&ok)[R],
+                                           const float* q, uint32_t dp, uint32_t d, int t,
+                                           float (&dist)[R][QB]) {
+  float acc[R][QB][4];
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int b = 0; b < QB; ++b)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[j][b][c] = 0.f;
+
+  const uint32_t nb16 = D16 > 0 ? uint32_t(D16) : d >> 4;
+#pragma unroll 4
+  for (uint32_t s = 0; s < nb16; ++s) {
+    float4 xv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      xv[j] = ok[j] ? *reinterpret_cast<const float4*>(xr[j] + 16 * s + 4 * t)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const float4 qv = *reinterpret_cast<const float4*>(q + b * dp + 16 * s + 4 * t);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        acc[j][b][0] = fma_term<L2>(qv.x, xv[j].x, acc[j][b][0]);
+        acc[j][b][1] = fma_term<L2>(qv.y, xv[j].y, acc[j][b][1]);
+        acc[j][b][2] = fma_term<L2>(qv.z, xv[j].z, acc[j][b][2]);
+        acc[j][b][3] = fma_term<L2>(qv.w, xv[j].w, acc[j][b][3]);
+      }
+    }
+  }
+  if (D16 == 0 && (d & 8)) {  // the 8-dim step adds into acc0 only (distance.h:64-65)
+    const uint32_t s = nb16;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const float4 xv = (ok[j] && t < 2) ? *reinterpret_cast<const float4*>(xr[j] + 16 * s + 4 * t)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t < 2) {
+#pragma unroll
+        for (int b = 0; b < QB; ++b) {
+          const float4 qv = *reinterpret_cast<const float4*>(q + b * dp + 16 * s + 4 * t);
+          acc[j][b][0] = fma_term<L2>(qv.x, xv.x, acc[j][b][0]);
+          acc[j][b][1] = fma_term<L2>(qv.y, xv.y, acc[j][b][1]);
+          acc[j][b][2] = fma_term<L2>(qv.z, xv.z, acc[j][b][2]);
+          acc[j][b][3] = fma_term<L2>(qv.w, xv.w, acc[j][b][3]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      float v[4], w[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] = acc[j][b][c] + dpp_f<kDppXor2>(acc[j][b][c]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) w[c] = v[c] + dpp_f<kDppXor1>(v[c]);
+      float r = (w[0] + w[1]) + (w[2] + w[3]);
+      if (D16 == 0 && (d & 7)) r = scalar_tail<L2>(q + b * dp, xr[j], int(d & ~7u), int(d), r);
+      dist[j][b] = L2 ? r : 1.0f - r;
+    }
+  }
+}
+
+// Offer keys of the calling wave to a query's k-list; `key` is kPadKey on lanes that carry no
+// candidate. Returns the updated admission threshold (the list's k-th key).
+template <int KP>
+__device__ __forceinline__ uint64_t offer(WaveList<KP>& lst, uint64_t thr, uint64_t key, int lane,
+                                          int k) {
+  uint64_t m = ballot(key < thr);
+  while (m) {
+    const int src = __builtin_ctzll(m);
+    m &= m - 1;
+    const uint64_t ck = readlane64(key, src);
+    if (ck < thr) thr = lst.insert(ck, lane, k);
+  }
+  return thr;
+}
+
+}  // namespace vsg

@@ -1,0 +1,373 @@
+// Filtered exact scan over allowed-id lists, for a batch of queries with different filters
+// and different k, in one persistent launch.
+//
+// Reference behaviour: HNSWIndex::search with SearchFilter (hnsw.h:25-33) takes the exact path
+// when n_allowed <= exact_below (hnsw.cpp:322-334) and scans the allowed rows with distance()
+// into a bounded heap (hnsw.cpp:336-354); search_batch pads missing results with kNone / +inf
+// (hnsw.cpp:356-366). Here every query carries its own allowed set (a CSR range of row ids) and
+// its own k; distances are the reference's bits (quad_dists) and ties go to the lower id.
+//
+// Work decomposition: queries whose CSR ranges are identical form a group and share every row
+// read. A work item is (group, chunk of <= C allowed rows, block of <= QB of the group's
+// queries). Waves pull items from a global counter until none is left, keep a k-list per
+// query, and write it to the query's slot for that chunk; the merge kernel then reduces the
+// chunk lists of each query.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "vsg.h"
+#include "vsg_internal.h"
+#include "vsg_kernels.h"
+
+namespace vsg {
+
+struct ListItem {
+  uint32_t lb, le;  // range of allow_ids
+  uint32_t chunk;   // chunk index within the group
+  uint32_t qoff;    // first entry in qperm
+  uint32_t nq;      // queries in this block
+};
+
+template <int QB, int R, int D16, int KP, bool L2>
+__global__ __launch_bounds__(kBlock) void lists_scan_kernel(
+    const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ queries,
+    const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, uint32_t n_items,
+    const uint32_t* __restrict__ qperm, const uint32_t* __restrict__ allow_ids,
+    const uint64_t* __restrict__ slot_off, uint32_t L, uint64_t* __restrict__ part,
+    uint32_t* __restrict__ counter) {
+  extern __shared__ float lds[];
+  const int tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
+  float* qw = lds + (tid >> 6) * QB * dp;  // this wave's queries
+
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const ListItem w = items[it];
+
+    // previous item's LDS reads are done before the queries are replaced
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (uint32_t i = lane; i < uint32_t(QB) * dp; i += 64) {
+      const uint32_t b = i / dp, c = i - b * dp;
+      qw[i] = (b < w.nq && c < d) ? queries[size_t(qperm[w.qoff + b]) * d + c] : 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+    WaveList<KP> lst[QB];
+    uint64_t thr[QB];
+    int kq[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      lst[b].init();
+      kq[b] = uint32_t(b) < w.nq ? int(min(ks[qperm[w.qoff + b]], uint32_t(KP))) : 0;
+      thr[b] = kq[b] > 0 ? kPadKey : 0;  // k == 0 admits nothing
+    }
+
+    for (uint32_t r0 = w.lb; r0 < w.le; r0 += 16 * R) {
+      uint32_t row[R];
+      bool ok[R];
+      const float* xr[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint32_t idx = r0 + j * 16 + g;
+        ok[j] = idx < w.le;
+        row[j] = allow_ids[ok[j] ? idx : w.lb];
+        xr[j] = base + size_t(row[j]) * dp;
+      }
+      float dist[R][QB];
+      quad_dists<QB, R, D16, L2>(xr, ok, qw, dp, d, t, dist);
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int b = 0; b < QB; ++b) {
+          if (kq[b] == 0) continue;
+          const uint64_t key = (ok[j] && t == 0) ? make_key(dist[j][b], row[j]) : kPadKey;
+          thr[b] = offer<KP>(lst[b], thr[b], key, lane, kq[b]);
+        }
+    }
+
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      if (uint32_t(b) >= w.nq) continue;
+      const uint32_t q = qperm[w.qoff + b];
+      uint64_t* dst = part + (slot_off[q] + w.chunk) * L;
+#pragma unroll
+      for (int r = 0; r < KP / 64; ++r) {
+        const uint32_t s = r * 64 + lane;
+        if (s < L) dst[s] = lst[b].e[r];
+      }
+    }
+  }
+}
+
+// ---- host side ---------------------------------------------------------------------------
+
+namespace {
+
+enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut };
+
+hipError_t dev_reserve(ListsWorkspace& ws, int i, size_t bytes) {
+  if (bytes <= ws.dev_cap[i]) return hipSuccess;
+  if (ws.dev[i]) (void)hipFree(ws.dev[i]);
+  ws.dev[i] = nullptr;
+  ws.dev_cap[i] = 0;
+  const size_t want = std::max(bytes, size_t(4096));
+  hipError_t e = hipMalloc(&ws.dev[i], want);
+  if (e == hipSuccess) ws.dev_cap[i] = want;
+  return e;
+}
+
+hipError_t host_reserve(ListsWorkspace& ws, int i, size_t bytes) {
+  if (bytes <= ws.host_cap[i]) return hipSuccess;
+  if (ws.host[i]) (void)hipHostFree(ws.host[i]);
+  ws.host[i] = nullptr;
+  ws.host_cap[i] = 0;
+  const size_t want = std::max(bytes, size_t(4096));
+  hipError_t e = hipHostMalloc(&ws.host[i], want, hipHostMallocDefault);
+  if (e == hipSuccess) ws.host_cap[i] = want;
+  return e;
+}
+
+struct ListsArgs {
+  dim3 grid;
+  size_t lds;
+  hipStream_t st;
+  const float* base;
+  uint32_t dp, d;
+  const float* q;
+  const uint32_t* ks;
+  const ListItem* items;
+  uint32_t n_items;
+  const uint32_t* qperm;
+  const uint32_t* ids;
+  const uint64_t* slot_off;
+  uint32_t L;
+  uint64_t* part;
+  uint32_t* counter;
+};
+
+template <int QB, int R, int D16, int KP, bool L2>
+hipError_t launch_lists_t(const ListsArgs& a) {
+  hipLaunchKernelGGL((lists_scan_kernel<QB, R, D16, KP, L2>), a.grid, dim3(kBlock), a.lds, a.st,
+                     a.base, a.dp, a.d, a.q, a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off,
+                     a.L, a.part, a.counter);
+  return hipGetLastError();
+}
+
+template <int QB, int R, int D16>
+hipError_t launch_lists_k(bool l2, int kp, const ListsArgs& a) {
+  if (kp == 64) return l2 ? launch_lists_t<QB, R, D16, 64, true>(a) : launch_lists_t<QB, R, D16, 64, false>(a);
+  return l2 ? launch_lists_t<QB, R, D16, 128, true>(a) : launch_lists_t<QB, R, D16, 128, false>(a);
+}
+
+template <int QB, int R>
+hipError_t launch_lists_d(bool l2, int kp, const ListsArgs& a) {
+  switch (a.d) {
+    case 384: return launch_lists_k<QB, R, 24>(l2, kp, a);
+    case 768: return launch_lists_k<QB, R, 48>(l2, kp, a);
+    case 128: return launch_lists_k<QB, R, 8>(l2, kp, a);
+    default: return launch_lists_k<QB, R, 0>(l2, kp, a);
+  }
+}
+
+std::string hip_msg(const char* what, hipError_t e) {
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+int hip_status(hipError_t e) {
+  return e == hipErrorOutOfMemory ? VSG_ERR_OUT_OF_MEMORY : VSG_ERR_DEVICE;
+}
+
+}  // namespace
+
+void ListsWorkspace::release() {
+  for (int i = 0; i < 8; ++i) {
+    if (dev[i]) (void)hipFree(dev[i]);
+    dev[i] = nullptr;
+    dev_cap[i] = 0;
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (host[i]) (void)hipHostFree(host[i]);
+    host[i] = nullptr;
+    host_cap[i] = 0;
+  }
+  if (counter) (void)hipFree(counter);
+  counter = nullptr;
+}
+
+#define LISTS_HIP(expr)                        \
+  do {                                         \
+    hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) {                    \
+      err = hip_msg(#expr, e_);                \
+      return hip_status(e_);                   \
+    }                                          \
+  } while (0)
+
+int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t dim, size_t dp,
+                      int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
+                      const uint32_t* ks, size_t kmax, const uint64_t* range,
+                      const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
+                      float* out_dists, std::string& err) {
+  // ---- validate (hnsw.cpp:326: an empty allowed set returns nothing) ----
+  const uint64_t total = n_ids;
+  for (size_t i = 0; i < nq; ++i) {
+    if (range[2 * i + 1] < range[2 * i] || range[2 * i + 1] > n_ids) {
+      err = "each query's range must satisfy begin <= end <= len(allow_ids)";
+      return VSG_ERR_INVALID_ARGUMENT;
+    }
+    if (ks[i] > kmax) {
+      err = "ks[i] must be <= kmax";
+      return VSG_ERR_INVALID_ARGUMENT;
+    }
+  }
+  if (total > 0 && !allow_ids) {
+    err = "null allow_ids";
+    return VSG_ERR_INVALID_ARGUMENT;
+  }
+  // Group queries by identical range; each group's ids must be increasing and < n.
+  std::map<std::pair<uint64_t, uint64_t>, std::vector<uint32_t>> groups;
+  for (size_t i = 0; i < nq; ++i) groups[{range[2 * i], range[2 * i + 1]}].push_back(uint32_t(i));
+  for (const auto& kv : groups) {
+    const uint64_t lb = kv.first.first, le = kv.first.second;
+    for (uint64_t p = lb; p < le; ++p) {
+      if (allow_ids[p] >= n || (p > lb && allow_ids[p] <= allow_ids[p - 1])) {
+        err = "each query's allowed ids must be strictly increasing and < size()";
+        return VSG_ERR_INVALID_ARGUMENT;
+      }
+    }
+  }
+  if (total >= (uint64_t(1) << 32)) {
+    err = "too many allowed ids in one call (limit 2^32 - 1)";
+    return VSG_ERR_LENGTH;
+  }
+
+  // ---- plan: chunks and work items ----
+  // queries per item: 4 waves x qb x dp floats of LDS must fit in 64 KB
+  int qb = 8;
+  while (qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > 65536) qb /= 2;
+  if (size_t(kWavesPerBlock) * qb * dp * sizeof(float) > 65536) {
+    err = "dim too large for the filtered scan (at most 4096)";
+    return VSG_ERR_INVALID_ARGUMENT;
+  }
+  // chunk rows: aim for ~8 items per wave slot, within [256, 8192], and at most
+  // kMaxListsPerQuery chunks per query
+  uint64_t visits = 0;
+  for (const auto& kv : groups) visits += (kv.first.second - kv.first.first) * ((kv.second.size() + qb - 1) / qb);
+  const uint64_t wave_slots = uint64_t(num_cus) * 16;
+  uint64_t C = visits / (wave_slots * 8) + 1;
+  C = std::min<uint64_t>(8192, std::max<uint64_t>(256, (C + 63) / 64 * 64));
+  uint64_t max_len = 0;
+  for (const auto& kv : groups) max_len = std::max(max_len, kv.first.second - kv.first.first);
+  while ((max_len + C - 1) / C > kMaxListsPerQuery) C *= 2;
+
+  std::vector<ListItem> items;
+  std::vector<uint32_t> qperm;
+  qperm.reserve(nq);
+  std::vector<uint64_t> slot_off(nq + 1, 0);
+  std::vector<uint32_t> nchunks_of(nq, 0);
+  for (const auto& kv : groups) {
+    const uint64_t lb = kv.first.first, le = kv.first.second;
+    const uint32_t nch = uint32_t((le - lb + C - 1) / C);
+    const auto& qs = kv.second;
+    for (uint32_t q : qs) nchunks_of[q] = nch;
+    for (size_t b0 = 0; b0 < qs.size(); b0 += qb) {
+      const uint32_t qoff = uint32_t(qperm.size());
+      const uint32_t nb = uint32_t(std::min<size_t>(qb, qs.size() - b0));
+      for (uint32_t j = 0; j < nb; ++j) qperm.push_back(qs[b0 + j]);
+      for (uint32_t c = 0; c < nch; ++c) {
+        ListItem it;
+        it.lb = uint32_t(lb + uint64_t(c) * C);
+        it.le = uint32_t(std::min<uint64_t>(le, lb + uint64_t(c + 1) * C));
+        it.chunk = c;
+        it.qoff = qoff;
+        it.nq = nb;
+        items.push_back(it);
+      }
+    }
+  }
+  for (size_t i = 0; i < nq; ++i) slot_off[i + 1] = slot_off[i] + nchunks_of[i];
+  const uint32_t L = uint32_t(kmax);
+  const uint64_t n_slots = slot_off[nq];
+
+  // ---- upload ----
+  const size_t qbytes = nq * dim * sizeof(float);
+  LISTS_HIP(dev_reserve(ws, kDQ, qbytes));
+  LISTS_HIP(dev_reserve(ws, kDKs, nq * sizeof(uint32_t)));
+  LISTS_HIP(dev_reserve(ws, kDItems, std::max<size_t>(1, items.size()) * sizeof(ListItem)));
+  LISTS_HIP(dev_reserve(ws, kDQperm, nq * sizeof(uint32_t)));
+  LISTS_HIP(dev_reserve(ws, kDSlot, (nq + 1) * sizeof(uint64_t)));
+  LISTS_HIP(dev_reserve(ws, kDIds, std::max<uint64_t>(1, total) * sizeof(uint32_t)));
+  LISTS_HIP(dev_reserve(ws, kDPart, std::max<uint64_t>(1, n_slots) * L * sizeof(uint64_t)));
+  LISTS_HIP(dev_reserve(ws, kDOut, nq * kmax * (sizeof(uint32_t) + sizeof(float))));
+  LISTS_HIP(host_reserve(ws, 0, qbytes));
+  LISTS_HIP(host_reserve(ws, 1, nq * kmax * sizeof(uint32_t)));
+  LISTS_HIP(host_reserve(ws, 2, nq * kmax * sizeof(float)));
+  if (!ws.counter) LISTS_HIP(hipMalloc(&ws.counter, sizeof(uint32_t)));
+
+  std::memcpy(ws.host[0], queries, qbytes);
+  LISTS_HIP(hipMemcpyAsync(ws.dev[kDQ], ws.host[0], qbytes, hipMemcpyHostToDevice, st));
+  LISTS_HIP(hipMemcpyAsync(ws.dev[kDKs], ks, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  if (!items.empty())
+    LISTS_HIP(hipMemcpyAsync(ws.dev[kDItems], items.data(), items.size() * sizeof(ListItem),
+                             hipMemcpyHostToDevice, st));
+  LISTS_HIP(hipMemcpyAsync(ws.dev[kDQperm], qperm.data(), nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  LISTS_HIP(hipMemcpyAsync(ws.dev[kDSlot], slot_off.data(), (nq + 1) * sizeof(uint64_t),
+                           hipMemcpyHostToDevice, st));
+  if (total)
+    LISTS_HIP(hipMemcpyAsync(ws.dev[kDIds], allow_ids, total * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  LISTS_HIP(hipMemsetAsync(ws.counter, 0, sizeof(uint32_t), st));
+
+  uint32_t* d_ids = static_cast<uint32_t*>(ws.dev[kDOut]);
+  float* d_dists = reinterpret_cast<float*>(d_ids + nq * kmax);
+  if (!items.empty()) {
+    ListsArgs a;
+    const uint64_t want_blocks = (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
+    a.grid = dim3(unsigned(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * 4)));
+    a.lds = size_t(kWavesPerBlock) * qb * dp * sizeof(float);
+    a.st = st;
+    a.base = base;
+    a.dp = uint32_t(dp);
+    a.d = uint32_t(dim);
+    a.q = static_cast<const float*>(ws.dev[kDQ]);
+    a.ks = static_cast<const uint32_t*>(ws.dev[kDKs]);
+    a.items = static_cast<const ListItem*>(ws.dev[kDItems]);
+    a.n_items = uint32_t(items.size());
+    a.qperm = static_cast<const uint32_t*>(ws.dev[kDQperm]);
+    a.ids = static_cast<const uint32_t*>(ws.dev[kDIds]);
+    a.slot_off = static_cast<const uint64_t*>(ws.dev[kDSlot]);
+    a.L = L;
+    a.part = static_cast<uint64_t*>(ws.dev[kDPart]);
+    a.counter = ws.counter;
+    const bool l2 = metric == VSG_METRIC_L2;
+    const int kp = kmax <= 64 ? 64 : 128;
+    hipError_t e = qb == 8   ? launch_lists_d<8, 2>(l2, kp, a)
+                   : qb == 4 ? launch_lists_d<4, 2>(l2, kp, a)
+                   : qb == 2 ? launch_lists_d<2, 4>(l2, kp, a)
+                             : launch_lists_d<1, 4>(l2, kp, a);
+    LISTS_HIP(e);
+  }
+  LISTS_HIP(launch_merge(uint32_t(nq), static_cast<const uint64_t*>(ws.dev[kDPart]),
+                         static_cast<const uint64_t*>(ws.dev[kDSlot]), 0, L,
+                         static_cast<const uint32_t*>(ws.dev[kDKs]), 0, nullptr, d_ids, d_dists,
+                         uint32_t(kmax), st));
+  LISTS_HIP(hipMemcpyAsync(ws.host[1], d_ids, nq * kmax * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  LISTS_HIP(hipMemcpyAsync(ws.host[2], d_dists, nq * kmax * sizeof(float), hipMemcpyDeviceToHost, st));
+  LISTS_HIP(hipStreamSynchronize(st));
+  std::memcpy(out_ids, ws.host[1], nq * kmax * sizeof(uint32_t));
+  std::memcpy(out_dists, ws.host[2], nq * kmax * sizeof(float));
+  return VSG_OK;
+}
+
+}  // namespace vsg
