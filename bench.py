@@ -125,8 +125,11 @@ def main():
     maxb = max([B] + [int(b) for b in args.sweep.split(",") if b])
     ids = torch.empty((maxb, k), dtype=torch.int32, device=dev)
     dists = torch.empty((maxb, k), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    # a dedicated (default-priority) stream: kernels and timing events share it
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
+    assert sptr != 0
 
     def step(i, b):
         off = (i * b) % (nqt - b + 1)

@@ -96,6 +96,10 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
                      size_t kmax, const uint64_t* range, const uint32_t* allow_ids, size_t n_ids,
                      uint32_t* out_ids, float* out_dists);
 
+/* Launch-shape parameters (the tile table an autotuner fills). Known names:
+ *   "scan_waves_per_cu"  waves per CU of the streaming scan (1..32, default 8). */
+int vsg_set_param(vsg_index* h, const char* name, int64_t value);
+
 /* Kernel timing for measurement: while enabled, every search records HIP events around its
  * scan launches and its merge launches on the stream they run on. vsg_kernel_times waits for
  * the recorded events, returns the accumulated milliseconds and launch counts since the last

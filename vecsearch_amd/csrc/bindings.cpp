@@ -186,6 +186,13 @@ PYBIND11_MODULE(_vecsearch, m) {
              return d;
            },
            "Accumulated kernel milliseconds and launch counts since the last call (HIP events).")
+      .def(
+          "set_param",
+          [](GpuBruteForceIndex& self, const std::string& name, int64_t value) {
+            if (vsg_set_param(self.handle(), name.c_str(), value) != VSG_OK)
+              throw std::invalid_argument(vsg_last_error());
+          },
+          py::arg("name"), py::arg("value"), "Set a launch-shape parameter (see vsg.h).")
       .def("synchronize", [](GpuBruteForceIndex& self) {
         py::gil_scoped_release nogil;
         if (vsg_synchronize(self.handle()) != VSG_OK) throw std::runtime_error(vsg_last_error());

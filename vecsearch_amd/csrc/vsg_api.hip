@@ -102,7 +102,9 @@ struct vsg_index {
   int device = 0;
   int num_cus = 256;
   size_t n = 0, cap = 0;
-  float* base = nullptr;  // cap x dp, zero padded
+  float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
+  float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
+  int scan_waves_per_cu = 8;
   hipStream_t stream = nullptr;
   std::mutex mu;
 
@@ -132,6 +134,7 @@ struct vsg_index {
     }
     if (stream) (void)hipStreamSynchronize(stream);
     if (base) (void)hipFree(base);
+    if (tiles) (void)hipFree(tiles);
     q.release();
     part.release();
     ids.release();
@@ -147,21 +150,34 @@ struct vsg_index {
 
 namespace {
 
+// Capacity grows by doubling (gpu_brute_force.cu:410-415): new zeroed buffers, old rows copied.
+// cap is a multiple of 16 so the tiled copy holds whole tiles.
 int grow(vsg_index* h, size_t need) {
   if (need <= h->cap) return VSG_OK;
-  const size_t cap = std::max({need, h->cap * 2, size_t(1024)});
+  size_t cap = std::max({need, h->cap * 2, size_t(1024)});
+  cap = (cap + 15) / 16 * 16;
+  const size_t bytes = cap * h->dp * sizeof(float);
   float* nb = nullptr;
-  VSG_HIP(hipMalloc(&nb, cap * h->dp * sizeof(float)));
-  hipError_t e = hipMemsetAsync(nb, 0, cap * h->dp * sizeof(float), h->stream);
-  if (e == hipSuccess && h->n)
+  float* nt = nullptr;
+  hipError_t e = hipMalloc(&nb, bytes);
+  if (e == hipSuccess) e = hipMalloc(&nt, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(nb, 0, bytes, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(nt, 0, bytes, h->stream);
+  if (e == hipSuccess && h->n) {
     e = hipMemcpyAsync(nb, h->base, h->n * h->dp * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+    const size_t tiled = (h->n + 15) / 16 * 16 * h->dp * sizeof(float);
+    if (e == hipSuccess) e = hipMemcpyAsync(nt, h->tiles, tiled, hipMemcpyDeviceToDevice, h->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
-    (void)hipFree(nb);
+    if (nb) (void)hipFree(nb);
+    if (nt) (void)hipFree(nt);
     return fail_hip("grow", e);
   }
   if (h->base) VSG_HIP(hipFree(h->base));
+  if (h->tiles) VSG_HIP(hipFree(h->tiles));
   h->base = nb;
+  h->tiles = nt;
   h->cap = cap;
   return VSG_OK;
 }
@@ -174,8 +190,9 @@ struct ScanPlan {
 
 ScanPlan plan_scan(const vsg_index* h) {
   const size_t n = h->n;
-  // Enough waves to keep every CU streaming (16 waves per CU), and at least 64 rows per wave.
-  const size_t max_waves = std::min<size_t>(size_t(h->num_cus) * 16, vsg::kMaxListsPerQuery);
+  // scan_waves_per_cu waves per CU (8 measured best at 1M x 384), at least 64 rows per wave.
+  const size_t max_waves =
+      std::min<size_t>(size_t(h->num_cus) * h->scan_waves_per_cu, vsg::kMaxListsPerQuery);
   size_t waves = std::min(max_waves, std::max<size_t>(1, (n + 63) / 64));
   size_t blocks = (waves + 3) / 4;
   size_t rpw = (n + blocks * 4 - 1) / (blocks * 4);
@@ -242,7 +259,7 @@ int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int meth
       TimedScope ts(h, st, 0);
       for (size_t q0 = 0; q0 < rq; q0 += qb) {
         const uint32_t nqp = uint32_t(std::min<size_t>(qb, rq - q0));
-        e = vsg::launch_scan(qb, l2, uint32_t(k), p.n_blocks, st, h->base, uint32_t(h->n),
+        e = vsg::launch_scan(qb, l2, uint32_t(k), p.n_blocks, st, h->tiles, uint32_t(h->n),
                              uint32_t(h->dp), uint32_t(h->dim), qr, uint32_t(q0), nqp,
                              p.rows_per_wave, d_mask, h->part.p);
         if (e != hipSuccess) return fail_hip("scan kernel", e);
@@ -367,6 +384,8 @@ int vsg_add(vsg_index* h, const float* data, size_t m) {
   if (rc) return rc;
   VSG_HIP(hipMemcpy2DAsync(h->base + h->n * h->dp, h->dp * sizeof(float), data, h->dim * sizeof(float),
                            h->dim * sizeof(float), m, hipMemcpyHostToDevice, h->stream));
+  VSG_HIP(vsg::launch_tile_rows(h->stream, h->base, h->tiles, uint32_t(h->n), uint32_t(h->n + m),
+                                uint32_t(h->dp)));
   VSG_HIP(hipStreamSynchronize(h->stream));
   h->n += m;
   return VSG_OK;
@@ -383,11 +402,31 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
     fill_missing(out_ids, out_dists, nq * k);
     return VSG_OK;
   }
-  if (mask && std::none_of(mask, mask + h->n, [](uint8_t v) { return v != 0; })) {
-    fill_missing(out_ids, out_dists, nq * k);  // hnsw.cpp:326: no allowed row, no result
-    return VSG_OK;
-  }
   VSG_HIP(hipSetDevice(h->device));
+  if (mask) {
+    const size_t allowed = size_t(std::count_if(mask, mask + h->n, [](uint8_t v) { return v != 0; }));
+    if (allowed == 0) {
+      fill_missing(out_ids, out_dists, nq * k);  // hnsw.cpp:326: no allowed row, no result
+      return VSG_OK;
+    }
+    if (allowed * 4 < h->n) {
+      // Sparse filter: gather the allowed rows (the exact_below regime) instead of streaming
+      // the whole base; every query shares the one id list.
+      std::vector<uint32_t> ids;
+      ids.reserve(allowed);
+      for (size_t i = 0; i < h->n; ++i)
+        if (mask[i]) ids.push_back(uint32_t(i));
+      std::vector<uint64_t> range(2 * nq);
+      for (size_t i = 0; i < nq; ++i) {
+        range[2 * i] = 0;
+        range[2 * i + 1] = allowed;
+      }
+      std::vector<uint32_t> ks(nq, uint32_t(k));
+      return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
+                                    h->stream, queries, nq, ks.data(), k, range.data(), ids.data(),
+                                    ids.size(), out_ids, out_dists, g_err);
+    }
+  }
   hipError_t e;
   if ((e = h->q.reserve(nq * h->dim)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->ids.reserve(nq * k)) != hipSuccess) return fail_hip("workspace", e);
@@ -443,6 +482,18 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
   return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
                                 h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids,
                                 out_ids, out_dists, g_err);
+}
+
+int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
+  if (!h || !name) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(h->mu);
+  const std::string key(name);
+  if (key == "scan_waves_per_cu") {
+    if (value < 1 || value > 32) return fail(VSG_ERR_INVALID_ARGUMENT, "scan_waves_per_cu must be in 1..32");
+    h->scan_waves_per_cu = int(value);
+    return VSG_OK;
+  }
+  return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
 }
 
 int vsg_kernel_timing(vsg_index* h, int enable) {

@@ -16,13 +16,15 @@ constexpr uint32_t kNoneId = 0xFFFFFFFFu;
 
 // vsg_scan.hip
 hipError_t launch_scan(int qb, bool l2, uint32_t k, uint32_t n_blocks, hipStream_t st,
-                       const float* base, uint32_t n, uint32_t dp, uint32_t d, const float* q,
+                       const float* tiles, uint32_t n, uint32_t dp, uint32_t d, const float* q,
                        uint32_t q0, uint32_t nqp, uint32_t rpw, const uint8_t* mask,
                        uint64_t* part);
 hipError_t launch_merge(uint32_t n_out, const uint64_t* part, const uint64_t* list_off, uint32_t W,
                         uint32_t L, const uint32_t* ks, uint32_t k, const uint32_t* qmap,
                         uint32_t* ids, float* dists, uint32_t ldo, hipStream_t st);
 hipError_t launch_fill_missing(hipStream_t st, uint32_t* ids, float* dists, size_t count);
+hipError_t launch_tile_rows(hipStream_t st, const float* base, float* tiles, uint32_t r0, uint32_t r1,
+                            uint32_t dp);
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {

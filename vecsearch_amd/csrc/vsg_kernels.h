@@ -97,25 +97,47 @@ __device__ __forceinline__ float fma_term(float q, float x, float acc) {
   return __builtin_fmaf(q, x, acc);  // brute_force.cpp:33-34
 }
 
+// Row addressing. A row's 16-dim step s starts at xr + s * STEP floats and element j sits at
+// xr + (j >> 4) * STEP + (j & 15):
+//   STEP = 16   row-major base (rows of dp floats);
+//   STEP = 256  16-row tiles, step-major: the 64-byte step-s blocks of a tile's 16 rows are
+//               contiguous, so a wave instruction over 16 rows reads 1 KB in one piece.
+// Load modes: kLoadIfOk skips rows with ok == false (masked scans), kLoadPlain and
+// kLoadStream always load (the caller clamps xr to a readable row); kLoadStream marks the
+// loads non-temporal (the base is streamed once per pass and should not displace L2 data).
+enum LoadMode { kLoadIfOk = 0, kLoadPlain = 1, kLoadStream = 2 };
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__device__ __forceinline__ float4 load_step(const float* p, bool ok) {
+  if (MODE == kLoadIfOk) return ok ? *reinterpret_cast<const float4*>(p) : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (MODE == kLoadStream) {
+    const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  return *reinterpret_cast<const float4*>(p);
+}
+
 // Scalar tail of d % 8 dims after the tree (distance.h:50, 68), as the reference's release
 // build computes it: one block of 4 rounded products added in order, then fused terms.
-template <bool L2>
+template <bool L2, int STEP>
 __device__ __forceinline__ float scalar_tail(const float* q, const float* __restrict__ x, int i,
                                              int d, float r) {
   if (i + 4 <= d) {
 #pragma unroll
     for (int j = 0; j < 4; ++j, ++i) {
-      const float a = q[i], b = x[i];
+      const float a = q[i], b = x[(i >> 4) * STEP + (i & 15)];
       const float p = L2 ? __fmul_rn(a - b, a - b) : __fmul_rn(a, b);
       r = __fadd_rn(r, p);
     }
   }
-  for (; i < d; ++i) r = fma_term<L2>(q[i], x[i], r);
+  for (; i < d; ++i) r = fma_term<L2>(q[i], x[(i >> 4) * STEP + (i & 15)], r);
   return r;
 }
 
 // Reference-order distances of R rows x QB queries for the calling quad lane t.
-// xr[j]: start of row j (row-major, dp floats, 64-byte aligned); ok[j] false skips the loads.
+// xr[j]: start of row j (see STEP); ok[j] marks rows that carry a result.
 // q: QB queries in LDS, query b at q + b * dp (zero beyond d). Every lane of a quad returns the
 // same dist[j][b]; all 64 lanes of the wave must call this together (DPP across the quad).
 //
@@ -123,7 +145,7 @@ __device__ __forceinline__ float scalar_tail(const float* q, const float* __rest
 // mod 16 (acc0 = accumulators 0..7, acc1 = 8..15 in distance.h:57-61). After the 16-dim steps
 // (and the 8-dim step into acc0), the tree of distance.h:18-27 runs across the quad with DPP:
 // v = acc + acc[lane ^ 2], w = v + v[lane ^ 1], r = (w0 + w1) + (w2 + w3).
-template <int QB, int R, int D16, bool L2>
+template <int QB, int R, int D16, bool L2, int STEP, int MODE>
 __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bool (&ok)[R],
                                            const float* q, uint32_t dp, uint32_t d, int t,
                                            float (&dist)[R][QB]) {
@@ -140,9 +162,7 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
   for (uint32_t s = 0; s < nb16; ++s) {
     float4 xv[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j)
-      xv[j] = ok[j] ? *reinterpret_cast<const float4*>(xr[j] + 16 * s + 4 * t)
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j = 0; j < R; ++j) xv[j] = load_step<MODE>(xr[j] + s * STEP + 4 * t, ok[j]);
 #pragma unroll
     for (int b = 0; b < QB; ++b) {
       const float4 qv = *reinterpret_cast<const float4*>(q + b * dp + 16 * s + 4 * t);
@@ -159,8 +179,8 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
     const uint32_t s = nb16;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const float4 xv = (ok[j] && t < 2) ? *reinterpret_cast<const float4*>(xr[j] + 16 * s + 4 * t)
-                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 xv = t < 2 ? load_step<MODE>(xr[j] + s * STEP + 4 * t, ok[j])
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
       if (t < 2) {
 #pragma unroll
         for (int b = 0; b < QB; ++b) {
@@ -183,7 +203,8 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
 #pragma unroll
       for (int c = 0; c < 4; ++c) w[c] = v[c] + dpp_f<kDppXor1>(v[c]);
       float r = (w[0] + w[1]) + (w[2] + w[3]);
-      if (D16 == 0 && (d & 7)) r = scalar_tail<L2>(q + b * dp, xr[j], int(d & ~7u), int(d), r);
+      if (D16 == 0 && (d & 7))
+        r = scalar_tail<L2, STEP>(q + b * dp, xr[j], int(d & ~7u), int(d), r);
       dist[j][b] = L2 ? r : 1.0f - r;
     }
   }

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
         xr[j] = base + size_t(row[j]) * dp;
       }
       float dist[R][QB];
-      quad_dists<QB, R, D16, L2>(xr, ok, qw, dp, d, t, dist);
+      quad_dists<QB, R, D16, L2, 16, kLoadPlain>(xr, ok, qw, dp, d, t, dist);
 #pragma unroll
       for (int j = 0; j < R; ++j)
 #pragma unroll

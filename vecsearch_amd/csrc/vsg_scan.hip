@@ -7,10 +7,12 @@
 // kernels, its distance matrix and its bitonic select passes (src/gpu_brute_force.cu:75-140,
 // 220-314, 442-482): no distance is ever written to HBM.
 //
-// Layout: the base is row-major in HBM with rows padded to dp = ceil(d/16)*16 floats (64-byte
-// aligned). A quad of lanes shares a row and each wave instruction reads 16 rows x 64
-// contiguous bytes (quad_dists in vsg_kernels.h). A lane works on R rows and QB queries per
-// step; the QB queries of a pass sit in LDS.
+// Layout: the scan reads the index's tiled copy of the base: 16-row tiles, rows padded to
+// dp = ceil(d/16)*16 floats, and inside a tile the 64-byte blocks of one 16-dim step of all 16
+// rows are contiguous. A quad of lanes shares a row, so each wave instruction reads one
+// contiguous 1 KB block (quad_dists in vsg_kernels.h, STEP = 256), with non-temporal loads:
+// the base is streamed once per pass. A lane works on R rows and QB queries per step; the QB
+// queries of a pass sit in LDS.
 //
 // Top-k: each wave keeps a sorted k-list per query in registers (WaveList). A row is admitted
 // only if its (dist, id) key beats the list's k-th key; admissions come from one ballot per
@@ -18,18 +20,20 @@
 // merge kernel picks the k smallest keys of a query over all its lists.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "vsg_internal.h"
 #include "vsg_kernels.h"
 
 namespace vsg {
 
 // One pass: queries [q0, q0 + nqp) of the batch (nqp <= QB) against the base. Wave gw scans
-// rows [gw * rpw, min(n, (gw + 1) * rpw)) and writes its k-list of each query of the pass to
-// part[(q * W + gw) * k + slot]. mask (nullable): rows with mask[row] == 0 are skipped without
-// being read. D16 > 0 specialises d = 16 * D16; D16 == 0 handles any d.
+// rows [gw * rpw, min(n, (gw + 1) * rpw)) (rpw a multiple of 64) and writes its k-list of each
+// query of the pass to part[(q * W + gw) * k + slot]. mask (nullable): rows with
+// mask[row] == 0 carry no result. D16 > 0 specialises d = 16 * D16; D16 == 0 handles any d.
 template <int QB, int R, int D16, int KP, bool L2>
 __global__ __launch_bounds__(kBlock) void scan_exact_kernel(
-    const float* __restrict__ base, uint32_t n, uint32_t dp, uint32_t d,
+    const float* __restrict__ tiles, uint32_t n, uint32_t dp, uint32_t d,
     const float* __restrict__ queries, uint32_t q0, uint32_t nqp, uint32_t k, uint32_t rpw,
     uint32_t W, const uint8_t* __restrict__ mask, uint64_t* __restrict__ part) {
   extern __shared__ float q_lds[];  // QB x dp
@@ -58,11 +62,13 @@ __global__ __launch_bounds__(kBlock) void scan_exact_kernel(
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       row[j] = r0 + j * 16 + g;
-      ok[j] = row[j] < row_end && (mask == nullptr || mask[row[j]] != 0);
-      xr[j] = base + size_t(ok[j] ? row[j] : row_begin) * dp;
+      const bool in = row[j] < row_end;
+      ok[j] = in && (mask == nullptr || mask[row[j]] != 0);
+      const uint32_t rr = in ? row[j] : row_begin;  // readable row for lanes past the end
+      xr[j] = tiles + size_t(rr >> 4) * 16 * dp + (rr & 15) * 16;
     }
     float dist[R][QB];
-    quad_dists<QB, R, D16, L2>(xr, ok, q_lds, dp, d, t, dist);
+    quad_dists<QB, R, D16, L2, 256, kLoadStream>(xr, ok, q_lds, dp, d, t, dist);
 #pragma unroll
     for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -120,19 +126,19 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
     od[c] = __builtin_inff();
   }
 
+  // heads of this thread's lists stay in registers; advancing a list loads one new key
   uint32_t pos[kMergeHeads];
+  uint64_t head[kMergeHeads];
   uint64_t best = kPadKey;
   int best_h = -1;
 #pragma unroll
   for (int h = 0; h < kMergeHeads; ++h) {
     pos[h] = 0;
     const uint32_t l = tid + h * 256;
-    if (l < nl && L > 0) {
-      const uint64_t v = lists[size_t(l) * L];
-      if (v < best) {
-        best = v;
-        best_h = h;
-      }
+    head[h] = (l < nl && L > 0) ? lists[size_t(l) * L] : kPadKey;
+    if (head[h] < best) {
+      best = head[h];
+      best_h = h;
     }
   }
   for (uint32_t r = 0; r < kq; ++r) {
@@ -159,18 +165,28 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
       int nh = -1;
 #pragma unroll
       for (int h = 0; h < kMergeHeads; ++h) {
-        if (h == best_h) pos[h]++;
-        const uint32_t ll = tid + h * 256;
-        if (ll < nl && pos[h] < L) {
-          const uint64_t v = lists[size_t(ll) * L + pos[h]];
-          if (v < best) {
-            best = v;
-            nh = h;
-          }
+        if (h == best_h) {
+          pos[h]++;
+          head[h] = pos[h] < L ? lists[size_t(tid + h * 256) * L + pos[h]] : kPadKey;
+        }
+        if (head[h] < best) {
+          best = head[h];
+          nh = h;
         }
       }
       best_h = nh;
     }
+  }
+}
+
+// Writes rows [r0, r1) of the row-major base into the tiled copy (see the layout note above).
+__global__ void tile_rows_kernel(const float* __restrict__ base, float* __restrict__ tiles,
+                                 uint32_t r0, uint32_t r1, uint32_t dp) {
+  const size_t total = size_t(r1 - r0) * dp;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const uint32_t r = r0 + uint32_t(i / dp), j = uint32_t(i % dp);
+    tiles[size_t(r >> 4) * 16 * dp + (j >> 4) * 256 + (r & 15) * 16 + (j & 15)] = base[size_t(r) * dp + j];
   }
 }
 
@@ -191,7 +207,7 @@ struct ScanArgs {
   dim3 grid;
   size_t lds;
   hipStream_t st;
-  const float* base;
+  const float* tiles;
   uint32_t n, dp, d;
   const float* q;
   uint32_t q0, nqp, k, rpw, W;
@@ -202,7 +218,7 @@ struct ScanArgs {
 template <int QB, int R, int D16, int KP, bool L2>
 hipError_t launch_scan_t(const ScanArgs& a) {
   hipLaunchKernelGGL((scan_exact_kernel<QB, R, D16, KP, L2>), a.grid, dim3(kBlock), a.lds, a.st,
-                     a.base, a.n, a.dp, a.d, a.q, a.q0, a.nqp, a.k, a.rpw, a.W, a.mask, a.part);
+                     a.tiles, a.n, a.dp, a.d, a.q, a.q0, a.nqp, a.k, a.rpw, a.W, a.mask, a.part);
   return hipGetLastError();
 }
 
@@ -225,14 +241,14 @@ hipError_t launch_scan_d(bool l2, int kp, const ScanArgs& a) {
 }  // namespace
 
 hipError_t launch_scan(int qb, bool l2, uint32_t k, uint32_t n_blocks, hipStream_t st,
-                       const float* base, uint32_t n, uint32_t dp, uint32_t d, const float* q,
+                       const float* tiles, uint32_t n, uint32_t dp, uint32_t d, const float* q,
                        uint32_t q0, uint32_t nqp, uint32_t rpw, const uint8_t* mask,
                        uint64_t* part) {
   ScanArgs a;
   a.grid = dim3(n_blocks);
   a.lds = size_t(qb) * dp * sizeof(float);
   a.st = st;
-  a.base = base;
+  a.tiles = tiles;
   a.n = n;
   a.dp = dp;
   a.d = d;
@@ -259,6 +275,15 @@ hipError_t launch_merge(uint32_t n_out, const uint64_t* part, const uint64_t* li
   if (n_out == 0) return hipSuccess;
   hipLaunchKernelGGL(merge_lists_kernel, dim3(n_out), dim3(256), 0, st, part, list_off, W, L, ks, k,
                      qmap, ids, dists, ldo);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_rows(hipStream_t st, const float* base, float* tiles, uint32_t r0, uint32_t r1,
+                            uint32_t dp) {
+  if (r1 <= r0) return hipSuccess;
+  const size_t total = size_t(r1 - r0) * dp;
+  const unsigned blocks = unsigned(std::min<size_t>(8192, (total + 255) / 256));
+  hipLaunchKernelGGL(tile_rows_kernel, dim3(blocks), dim3(256), 0, st, base, tiles, r0, r1, dp);
   return hipGetLastError();
 }
 
