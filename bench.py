@@ -157,7 +157,7 @@ def main():
     log(f"batch {B}: {ms_per_step:.3f} ms/step, {value:,.0f} QPS; kernels {kt}")
 
     qb = 1 if B == 1 or args.method == "naive" else 2 if B == 2 else 4 if B <= 4 else 8
-    scan_ms = kt["scan_ms"] / max(1, kt["scan_launches"])
+    scan_ms = kt["scan_ms"] / max(1, kt["scan_launches"])  # (exact path; shortlist: see kernels)
     alg_bytes = n * d * 4 + qb * d * 4 + qb * k * 8
     alg_flops = 2.0 * n * d * qb
     t_hbm, t_fp = alg_bytes / (HBM_PEAK_GBS * 1e9), alg_flops / (FP32_PEAK_TFLOPS * 1e12)

@@ -22,8 +22,9 @@ enum class Metric : uint32_t { L2 = 0, InnerProduct = 1 };
 
 // gpu_brute_force.h:16-21. Every method returns the same ids and distances; they differ in
 // how the scan is run: Naive = one query per streaming pass, Skinny = up to 8 queries per
-// streaming pass, Tiled = the shortlist scan for large batches, Auto = by batch size.
-enum class GpuMethod : uint32_t { Auto = 0, Naive = 1, Skinny = 2, Tiled = 3 };
+// streaming pass, Tiled / Bf16 = the certified BF16 MFMA shortlist, Fp8 = the certified FP8
+// shortlist, Auto = by batch size (see VSG_METHOD_* in vsg.h).
+enum class GpuMethod : uint32_t { Auto = 0, Naive = 1, Skinny = 2, Tiled = 3, Bf16 = 4, Fp8 = 5 };
 
 // gpu_brute_force.h:23-24: true if at least one GPU is visible (HIP runtime). The name is kept
 // for callers of the reference API; hip_available() is the same test.

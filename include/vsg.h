@@ -37,13 +37,22 @@ typedef struct vsg_index vsg_index;
 #define VSG_METRIC_L2 0
 #define VSG_METRIC_IP 1
 
-/* Search method (gpu_brute_force.h:16-21). AUTO picks by batch size; NAIVE, SKINNY and TILED
- * force the one-query streaming scan, the multi-query streaming scan and the MFMA shortlist
- * scan respectively. All of them return the same ids and distances. */
+/* Search method (gpu_brute_force.h:16-21). Every method returns the same ids and distances
+ * (the reference CPU index's); they differ only in how the work is done:
+ *   NAIVE   the exact streaming scan, one query per pass over the base;
+ *   SKINNY  the exact streaming scan, up to 8 queries per pass;
+ *   TILED   the certified shortlist with a BF16 MFMA scan (large batches);
+ *   BF16    the same as TILED;
+ *   FP8     the certified shortlist with an FP8 E4M3 (per-row scale) MFMA scan;
+ *   AUTO    SKINNY for small batches, TILED for large ones.
+ * The shortlist applies to inner product / cosine with dim <= 384 on bases of at least
+ * 65,536 rows; elsewhere every method runs the exact scan. */
 #define VSG_METHOD_AUTO 0
 #define VSG_METHOD_NAIVE 1
 #define VSG_METHOD_SKINNY 2
 #define VSG_METHOD_TILED 3
+#define VSG_METHOD_BF16 4
+#define VSG_METHOD_FP8 5
 
 #define VSG_NONE 0xFFFFFFFFu /* gpu_brute_force.h:28 kNone */
 #define VSG_MAX_K 128        /* gpu_brute_force.h:29 kMaxK */
@@ -97,16 +106,25 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
                      uint32_t* out_ids, float* out_dists);
 
 /* Launch-shape parameters (the tile table an autotuner fills). Known names:
- *   "scan_waves_per_cu"  waves per CU of the streaming scan (1..32, default 8). */
+ *   "scan_waves_per_cu"  waves per CU of the streaming scan (1..32, default 8);
+ *   "auto_short_batch"   smallest batch that AUTO sends to the shortlist (default 32). */
 int vsg_set_param(vsg_index* h, const char* name, int64_t value);
 
 /* Kernel timing for measurement: while enabled, every search records HIP events around its
- * scan launches and its merge launches on the stream they run on. vsg_kernel_times waits for
- * the recorded events, returns the accumulated milliseconds and launch counts since the last
- * call, and resets them. */
+ * kernel launches on the stream they run on, by kind. vsg_kernel_times waits for the recorded
+ * events, writes the accumulated milliseconds and launch counts per kind since the last call
+ * (arrays of VSG_KERNEL_KINDS entries), and resets them. */
+#define VSG_KERNEL_SCAN 0    /* exact streaming scan passes */
+#define VSG_KERNEL_MERGE 1   /* per-query merge of the scan's lists */
+#define VSG_KERNEL_MFMA 2    /* shortlist MFMA passes (tile minima, emit) */
+#define VSG_KERNEL_RESCORE 3 /* shortlist exact rescoring */
+#define VSG_KERNEL_KINDS 4
 int vsg_kernel_timing(vsg_index* h, int enable);
-int vsg_kernel_times(vsg_index* h, double* scan_ms, uint64_t* scan_launches, double* merge_ms,
-                     uint64_t* merge_launches);
+int vsg_kernel_times(vsg_index* h, double* ms, uint64_t* launches);
+
+/* Shortlist statistics of the last search that used it: candidates emitted (sum over queries),
+ * candidates rescored exactly (sum), and queries that overflowed and took the exact scan. */
+int vsg_short_stats(vsg_index* h, uint64_t* emitted, uint64_t* rescored, uint64_t* fallbacks);
 
 /* Wait for all work the index has enqueued on its own stream. */
 int vsg_synchronize(vsg_index* h);

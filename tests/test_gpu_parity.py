@@ -188,3 +188,66 @@ def test_mixture_100k_384(batch):
     gpu = vs.GpuBruteForceIndex(384, "cosine")
     gpu.add(x)
     assert_same(gpu.search(q, 10), oracle.search(x, q, 10, "ip", threads=8))
+
+
+# ---- certified shortlist (large batches): must equal the exact results bit for bit ---------
+
+@pytest.fixture(scope="module")
+def mixture_70k():
+    x = dg.mixture(70_000, 384, seed=40)
+    q = dg.mixture(300, 384, seed=41)
+    gpu = vs.GpuBruteForceIndex(384, "cosine")
+    gpu.add(x)
+    return x, q, gpu
+
+
+@pytest.mark.parametrize("method", ["tiled", "bf16", "fp8", "auto"])
+@pytest.mark.parametrize("k", [1, 10, 100, 128])
+def test_shortlist_matches_oracle(mixture_70k, method, k):
+    x, q, gpu = mixture_70k
+    exp = oracle.search(x, q, k, "ip", threads=8)
+    got = gpu.search(q, k, method=method)
+    assert_same(got, exp)
+
+
+@pytest.mark.parametrize("d", [100, 128, 256])
+def test_shortlist_other_dims(d):
+    x = dg.mixture(66_000, d, seed=42)
+    q = dg.mixture(70, d, seed=43)
+    gpu = vs.GpuBruteForceIndex(d, "ip")
+    gpu.add(x)
+    for method in ("bf16", "fp8"):
+        assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
+
+
+def test_shortlist_overflow_falls_back_exactly():
+    # 3,000 identical rows tie with each other: the candidate set exceeds the buffer and the
+    # queries near them must be answered by the exact scan, with ties to the lower id.
+    x = dg.mixture(70_000, 128, seed=44)
+    x[10_000:13_000] = x[5]
+    q = np.concatenate([x[5:6] * 0.999 + 0.001, dg.mixture(63, 128, seed=45)]).astype(np.float32)
+    q = vs.normalize(q)
+    gpu = vs.GpuBruteForceIndex(128, "cosine")
+    gpu.add(x)
+    got = gpu.search(q, 50, method="bf16")
+    assert_same(got, oracle.search(x, q, 50, "ip", threads=8))
+    assert gpu.short_stats()["fallbacks"] >= 1
+    assert list(got[0][0, :50]) == [5] + list(range(10_000, 10_049))
+
+
+def test_shortlist_grows_with_adds(mixture_70k):
+    # rows added after the low-precision copies exist are converted before the next search
+    x, q, _ = mixture_70k
+    gpu = vs.GpuBruteForceIndex(384, "cosine")
+    gpu.add(x[:66_000])
+    assert_same(gpu.search(q[:64], 10, method="bf16"), oracle.search(x[:66_000], q[:64], 10, "ip", threads=8))
+    gpu.add(x[66_000:])
+    assert_same(gpu.search(q[:64], 10, method="bf16"), oracle.search(x, q[:64], 10, "ip", threads=8))
+
+
+def test_l2_large_batch_uses_exact_path():
+    x = dg.mixture(66_000, 64, seed=46)
+    q = dg.mixture(64, 64, seed=47)
+    gpu = vs.GpuBruteForceIndex(64, "l2")
+    gpu.add(x)
+    assert_same(gpu.search(q, 10, method="bf16"), oracle.search(x, q, 10, "l2", threads=8))

@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("VSG_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["vsg_api.hip", "vsg_scan.hip", "vsg_lists.hip"]
+HIP_SOURCES = ["vsg_api.hip", "vsg_scan.hip", "vsg_lists.hip", "vsg_short.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -51,7 +51,9 @@ GpuMethod parse_method(const std::string& method) {
   if (method == "naive") return GpuMethod::Naive;
   if (method == "skinny") return GpuMethod::Skinny;
   if (method == "tiled") return GpuMethod::Tiled;
-  throw std::invalid_argument("method must be 'auto', 'naive', 'skinny' or 'tiled'");
+  if (method == "bf16") return GpuMethod::Bf16;
+  if (method == "fp8") return GpuMethod::Fp8;
+  throw std::invalid_argument("method must be 'auto', 'naive', 'skinny', 'tiled', 'bf16' or 'fp8'");
 }
 
 void check_2d(const FloatArray& a, size_t dim) {
@@ -174,18 +176,30 @@ PYBIND11_MODULE(_vecsearch, m) {
           py::arg("enable"))
       .def("kernel_times",
            [](GpuBruteForceIndex& self) {
-             double s = 0, m = 0;
-             uint64_t sl = 0, ml = 0;
-             if (vsg_kernel_times(self.handle(), &s, &sl, &m, &ml) != VSG_OK)
-               throw std::runtime_error(vsg_last_error());
+             double ms[VSG_KERNEL_KINDS];
+             uint64_t n[VSG_KERNEL_KINDS];
+             if (vsg_kernel_times(self.handle(), ms, n) != VSG_OK) throw std::runtime_error(vsg_last_error());
              py::dict d;
-             d["scan_ms"] = s;
-             d["scan_launches"] = sl;
-             d["merge_ms"] = m;
-             d["merge_launches"] = ml;
+             const char* names[VSG_KERNEL_KINDS] = {"scan", "merge", "mfma", "rescore"};
+             for (int i = 0; i < VSG_KERNEL_KINDS; ++i) {
+               d[(std::string(names[i]) + "_ms").c_str()] = ms[i];
+               d[(std::string(names[i]) + "_launches").c_str()] = n[i];
+             }
              return d;
            },
-           "Accumulated kernel milliseconds and launch counts since the last call (HIP events).")
+           "Accumulated kernel milliseconds and launch counts per kind since the last call (HIP events).")
+      .def("short_stats",
+           [](GpuBruteForceIndex& self) {
+             uint64_t e = 0, r = 0, f = 0;
+             if (vsg_short_stats(self.handle(), &e, &r, &f) != VSG_OK) throw std::runtime_error(vsg_last_error());
+             py::dict d;
+             d["emitted"] = e;
+             d["rescored"] = r;
+             d["fallbacks"] = f;
+             return d;
+           },
+           "Shortlist counters of the last search that used it (emitted / rescored candidates are\n"
+           "summed over queries only while kernel timing is on; fallbacks always).")
       .def(
           "set_param",
           [](GpuBruteForceIndex& self, const std::string& name, int64_t value) {

@@ -105,6 +105,27 @@ struct vsg_index {
   float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
   float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
   int scan_waves_per_cu = 8;
+  int auto_short_batch = 32;
+
+  // low-precision copies for the shortlist (index 1 = bf16, 2 = fp8), rebuilt lazily
+  struct Prec {
+    void* frag = nullptr;             // fragment layout, cap rows
+    float* scale = nullptr;           // fp8 per-row scale
+    double* norms = nullptr;          // per row |x|^2, |x^|^2, |x - x^|^2
+    unsigned long long* maxes = nullptr;  // max |x|, |x^|, |x - x^| (fp64 bits)
+    size_t rows = 0, cap = 0;
+  } prec[3];
+  int ks = 0;  // 32-dim k-steps of the copies (0: shortlist unsupported for this dim)
+
+  // shortlist workspace
+  DevBuf<uint8_t> s_qfrag;
+  DevBuf<float> s_qscale, s_E, s_thr;
+  DevBuf<double> s_qnorms;
+  DevBuf<uint32_t> s_tilemin, s_count, s_flags, s_rescored, s_nover;
+  DevBuf<uint64_t> s_cand;
+  HostBuf<uint32_t> s_hover;
+  uint64_t st_emitted = 0, st_rescored = 0, st_fallback = 0;
+  size_t st_nq = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
 
@@ -121,7 +142,7 @@ struct vsg_index {
   // kernel timing (vsg_kernel_timing)
   struct Timed {
     hipEvent_t a, b;
-    int kind;  // 0 scan, 1 merge
+    int kind;  // VSG_KERNEL_* (vsg.h)
     uint32_t launches;
   };
   bool timing = false;
@@ -135,6 +156,24 @@ struct vsg_index {
     if (stream) (void)hipStreamSynchronize(stream);
     if (base) (void)hipFree(base);
     if (tiles) (void)hipFree(tiles);
+    for (auto& p : prec) {
+      if (p.frag) (void)hipFree(p.frag);
+      if (p.scale) (void)hipFree(p.scale);
+      if (p.norms) (void)hipFree(p.norms);
+      if (p.maxes) (void)hipFree(p.maxes);
+    }
+    s_qfrag.release();
+    s_qscale.release();
+    s_E.release();
+    s_thr.release();
+    s_qnorms.release();
+    s_tilemin.release();
+    s_count.release();
+    s_flags.release();
+    s_rescored.release();
+    s_nover.release();
+    s_cand.release();
+    s_hover.release();
     q.release();
     part.release();
     ids.release();
@@ -183,7 +222,8 @@ int grow(vsg_index* h, size_t need) {
 }
 
 // Plan of one streaming scan over the base: n_blocks workgroups of 4 waves, each wave owning
-// rows_per_wave consecutive rows (a multiple of 64).
+// rows_per_wave consecutive rows (a multiple of 64); each block leaves one k-list per query
+// (W = n_blocks lists).
 struct ScanPlan {
   uint32_t n_blocks, rows_per_wave, W;
 };
@@ -201,7 +241,7 @@ ScanPlan plan_scan(const vsg_index* h) {
   ScanPlan p;
   p.n_blocks = uint32_t(blocks);
   p.rows_per_wave = uint32_t(rpw);
-  p.W = uint32_t(blocks * 4);
+  p.W = uint32_t(blocks);
   return p;
 }
 
@@ -241,9 +281,9 @@ struct TimedScope {
   }
 };
 
-// Enqueue a full search of nq device-resident queries on stream st.
-int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
-                   const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
+// Enqueue an exact streaming search of nq device-resident queries on stream st.
+int enqueue_exact(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+                  const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
   const ScanPlan p = plan_scan(h);
   const int qb = queries_per_pass(method, nq, h->dp);
   const size_t per_query = size_t(p.W) * k * sizeof(uint64_t);
@@ -256,7 +296,7 @@ int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int meth
     const size_t rq = std::min(round, nq - r0);
     const float* qr = d_q + r0 * h->dim;
     {
-      TimedScope ts(h, st, 0);
+      TimedScope ts(h, st, VSG_KERNEL_SCAN);
       for (size_t q0 = 0; q0 < rq; q0 += qb) {
         const uint32_t nqp = uint32_t(std::min<size_t>(qb, rq - q0));
         e = vsg::launch_scan(qb, l2, uint32_t(k), p.n_blocks, st, h->tiles, uint32_t(h->n),
@@ -267,7 +307,7 @@ int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int meth
       }
     }
     {
-      TimedScope ts(h, st, 1);
+      TimedScope ts(h, st, VSG_KERNEL_MERGE);
       e = vsg::launch_merge(uint32_t(rq), h->part.p, nullptr, p.W, uint32_t(k), nullptr,
                             uint32_t(k), nullptr, d_ids + r0 * k, d_dists + r0 * k, uint32_t(k), st);
       if (e != hipSuccess) return fail_hip("merge kernel", e);
@@ -277,13 +317,187 @@ int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int meth
   return VSG_OK;
 }
 
+// ---- certified shortlist (vsg_short.hip) ---------------------------------------------------
+
+bool short_supported(const vsg_index* h, size_t k) {
+  return h->metric == VSG_METRIC_IP && vsg::short_ks(uint32_t(h->dim)) > 0 && h->n >= 65536 &&
+         k <= VSG_MAX_K && size_t(vsg::short_cand_cap()) >= 4 * k;
+}
+
+// Quantized copy `p` (1 bf16, 2 fp8) covering every row of the base.
+int ensure_prec(vsg_index* h, int p) {
+  auto& c = h->prec[p];
+  const int ks = vsg::short_ks(uint32_t(h->dim));
+  if (c.frag && (c.cap != h->cap || ks != h->ks)) {  // the base grew: rebuild from scratch
+    (void)hipFree(c.frag);
+    if (c.scale) (void)hipFree(c.scale);
+    (void)hipFree(c.norms);
+    (void)hipFree(c.maxes);
+    c = vsg_index::Prec();
+  }
+  h->ks = ks;
+  if (!c.frag) {
+    const size_t chunk = p == 1 ? 1024 : 512;
+    const size_t bytes = h->cap / 16 * size_t(ks) * chunk;
+    VSG_HIP(hipMalloc(&c.frag, bytes));
+    VSG_HIP(hipMemsetAsync(c.frag, 0, bytes, h->stream));
+    if (p == 2) VSG_HIP(hipMalloc(&c.scale, h->cap * sizeof(float)));
+    VSG_HIP(hipMalloc(&c.norms, h->cap * 3 * sizeof(double)));
+    VSG_HIP(hipMalloc(&c.maxes, 3 * sizeof(unsigned long long)));
+    VSG_HIP(hipMemsetAsync(c.maxes, 0, 3 * sizeof(unsigned long long), h->stream));
+    c.cap = h->cap;
+    c.rows = 0;
+  }
+  if (c.rows < h->n) {
+    VSG_HIP(vsg::launch_quantize(h->stream, h->base, uint32_t(h->dp), uint32_t(h->dim), uint32_t(c.rows),
+                                 uint32_t(h->n), uint32_t(ks), p, c.frag, c.scale, c.norms));
+    VSG_HIP(vsg::launch_norm_max(h->stream, c.norms, uint32_t(c.rows), uint32_t(h->n), c.maxes));
+    c.rows = h->n;
+  }
+  return VSG_OK;
+}
+
+int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+                   const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st);
+
+// Shortlist search of nq device-resident queries (prec 1 bf16, 2 fp8). Synchronises the stream
+// once at the end to learn whether any query overflowed its candidate buffer; those queries are
+// answered again by the exact scan.
+int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, uint32_t* d_ids,
+                  float* d_dists, hipStream_t st) {
+  if (st != h->stream) {  // the precision copy is built on the index stream
+    VSG_HIP(hipStreamSynchronize(st));
+  }
+  int rc = ensure_prec(h, p);
+  if (rc) return rc;
+  if (st != h->stream) VSG_HIP(hipStreamSynchronize(h->stream));
+  const auto& c = h->prec[p];
+  const int ks = h->ks;
+  const uint32_t cap = uint32_t(vsg::short_cand_cap());
+  const uint32_t tile_rows = uint32_t(vsg::short_tile_rows());
+  const uint32_t n_tiles = uint32_t((h->n + tile_rows - 1) / tile_rows);
+  const size_t round = std::min<size_t>(nq, 4096);
+  const size_t chunk = p == 1 ? 1024 : 512;
+  const size_t qpad = (round + 63) / 64 * 64;
+  hipError_t e;
+  if ((e = h->s_qfrag.reserve(qpad / 16 * ks * chunk)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_qnorms.reserve(qpad * 3)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_tilemin.reserve(round * n_tiles)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_hover.reserve(1)) != hipSuccess) return fail_hip("pinned staging", e);
+  h->st_emitted = h->st_rescored = h->st_fallback = 0;
+  h->st_nq = 0;
+
+  std::vector<uint32_t> flags, counts, rescored;
+  for (size_t r0 = 0; r0 < nq; r0 += round) {
+    const uint32_t rq = uint32_t(std::min(round, nq - r0));
+    const float* qr = d_q + r0 * h->dim;
+    uint32_t* ids = d_ids + r0 * k;
+    float* dists = d_dists + r0 * k;
+    VSG_HIP(vsg::launch_quantize(st, qr, uint32_t(h->dim), uint32_t(h->dim), 0, rq, uint32_t(ks), p,
+                                 h->s_qfrag.p, h->s_qscale.p, h->s_qnorms.p));
+    VSG_HIP(vsg::launch_query_bound(st, h->s_qnorms.p, rq, p, uint32_t(32 * ks), c.maxes, h->s_E.p));
+    VSG_HIP(vsg::launch_zero_u32(st, h->s_count.p, rq));
+    VSG_HIP(vsg::launch_zero_u32(st, h->s_nover.p, 1));
+
+    vsg::ShortScanArgs a;
+    a.st = st;
+    a.KS = ks;
+    a.xfrag = c.frag;
+    a.xscale = c.scale;
+    a.n = uint32_t(h->n);
+    a.qfrag = h->s_qfrag.p;
+    a.qscale = h->s_qscale.p;
+    a.nq = rq;
+    a.n_qblocks = (rq + 63) / 64;
+    // about 16 wave tasks per CU, segments a multiple of the tile
+    const size_t want_seg = std::max<size_t>(1, size_t(h->num_cus) * 16 / a.n_qblocks);
+    size_t seg_rows = (h->n + want_seg - 1) / want_seg;
+    seg_rows = std::max<size_t>(tile_rows, (seg_rows + tile_rows - 1) / tile_rows * tile_rows);
+    a.seg_rows = uint32_t(seg_rows);
+    a.n_tasks = uint32_t(a.n_qblocks * ((h->n + seg_rows - 1) / seg_rows));
+    a.n_tiles = n_tiles;
+    a.tilemin = h->s_tilemin.p;
+    a.thr = h->s_thr.p;
+    a.cand = h->s_cand.p;
+    a.cap = cap;
+    a.count = h->s_count.p;
+    {
+      TimedScope ts(h, st, VSG_KERNEL_MFMA);
+      VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
+      ts.launches++;
+    }
+    VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, n_tiles, rq, nullptr, uint32_t(k), h->s_E.p,
+                                         h->s_thr.p));
+    {
+      TimedScope ts(h, st, VSG_KERNEL_MFMA);
+      VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
+      ts.launches++;
+    }
+    {
+      TimedScope ts(h, st, VSG_KERNEL_RESCORE);
+      VSG_HIP(vsg::launch_rescore(st, h->metric == VSG_METRIC_L2, h->base, uint32_t(h->dp), uint32_t(h->dim),
+                                  qr, rq, nullptr, uint32_t(k), h->s_E.p, h->s_cand.p, h->s_count.p, ids,
+                                  dists, uint32_t(k), h->s_flags.p, h->s_nover.p, h->s_rescored.p));
+      ts.launches++;
+    }
+    VSG_HIP(hipMemcpyAsync(h->s_hover.p, h->s_nover.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VSG_HIP(hipStreamSynchronize(st));
+    const uint32_t nover = h->s_hover.p[0];
+    if (h->timing || nover) {
+      counts.resize(rq);
+      rescored.resize(rq);
+      VSG_HIP(hipMemcpy(counts.data(), h->s_count.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      VSG_HIP(hipMemcpy(rescored.data(), h->s_rescored.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < rq; ++i) {
+        h->st_emitted += counts[i];
+        h->st_rescored += rescored[i];
+      }
+    }
+    h->st_nq += rq;
+    if (nover) {
+      // overflowed queries: exact scan, one query at a time into this round's output rows
+      flags.resize(rq);
+      VSG_HIP(hipMemcpy(flags.data(), h->s_flags.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < rq; ++i) {
+        if (!flags[i]) continue;
+        h->st_fallback++;
+        rc = enqueue_search(h, qr + size_t(i) * h->dim, 1, k, VSG_METHOD_SKINNY, nullptr, ids + size_t(i) * k,
+                            dists + size_t(i) * k, st);
+        if (rc) return rc;
+      }
+    }
+  }
+  return VSG_OK;
+}
+
 int check_search_args(const vsg_index* h, size_t k, int method) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   if (k > VSG_MAX_K)
     return fail(VSG_ERR_INVALID_ARGUMENT, "k must be <= " + std::to_string(VSG_MAX_K));
-  if (method < VSG_METHOD_AUTO || method > VSG_METHOD_TILED)
+  if (method < VSG_METHOD_AUTO || method > VSG_METHOD_FP8)
     return fail(VSG_ERR_INVALID_ARGUMENT, "unknown search method");
   return VSG_OK;
+}
+
+// The method -> path choice (see VSG_METHOD_* in vsg.h).
+int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+                   const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
+  if (d_mask == nullptr && short_supported(h, k)) {
+    int p = 0;
+    if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16) p = 1;
+    else if (method == VSG_METHOD_FP8) p = 2;
+    else if (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch)) p = 1;
+    if (p) return enqueue_short(h, d_q, nq, k, p, d_ids, d_dists, st);
+  }
+  return enqueue_exact(h, d_q, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
 void fill_missing(uint32_t* ids, float* dists, size_t count) {
@@ -493,6 +707,11 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
     h->scan_waves_per_cu = int(value);
     return VSG_OK;
   }
+  if (key == "auto_short_batch") {
+    if (value < 1) return fail(VSG_ERR_INVALID_ARGUMENT, "auto_short_batch must be >= 1");
+    h->auto_short_batch = int(value);
+    return VSG_OK;
+  }
   return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
 }
 
@@ -503,13 +722,12 @@ int vsg_kernel_timing(vsg_index* h, int enable) {
   return VSG_OK;
 }
 
-int vsg_kernel_times(vsg_index* h, double* scan_ms, uint64_t* scan_launches, double* merge_ms,
-                     uint64_t* merge_launches) {
+int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   std::lock_guard<std::mutex> g(h->mu);
   VSG_HIP(hipSetDevice(h->device));
-  double ms[2] = {0, 0};
-  uint64_t cnt[2] = {0, 0};
+  double ms[VSG_KERNEL_KINDS] = {0, 0, 0, 0};
+  uint64_t cnt[VSG_KERNEL_KINDS] = {0, 0, 0, 0};
   hipError_t err = hipSuccess;
   for (auto& t : h->timed) {
     float v = 0;
@@ -523,10 +741,19 @@ int vsg_kernel_times(vsg_index* h, double* scan_ms, uint64_t* scan_launches, dou
   }
   h->timed.clear();
   if (err != hipSuccess) return fail_hip("vsg_kernel_times", err);
-  if (scan_ms) *scan_ms = ms[0];
-  if (scan_launches) *scan_launches = cnt[0];
-  if (merge_ms) *merge_ms = ms[1];
-  if (merge_launches) *merge_launches = cnt[1];
+  for (int i = 0; i < VSG_KERNEL_KINDS; ++i) {
+    if (ms_out) ms_out[i] = ms[i];
+    if (launches_out) launches_out[i] = cnt[i];
+  }
+  return VSG_OK;
+}
+
+int vsg_short_stats(vsg_index* h, uint64_t* emitted, uint64_t* rescored, uint64_t* fallbacks) {
+  if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
+  std::lock_guard<std::mutex> g(h->mu);
+  if (emitted) *emitted = h->st_emitted;
+  if (rescored) *rescored = h->st_rescored;
+  if (fallbacks) *fallbacks = h->st_fallback;
   return VSG_OK;
 }
 

@@ -26,6 +26,42 @@ hipError_t launch_fill_missing(hipStream_t st, uint32_t* ids, float* dists, size
 hipError_t launch_tile_rows(hipStream_t st, const float* base, float* tiles, uint32_t r0, uint32_t r1,
                             uint32_t dp);
 
+// vsg_short.hip: certified low-precision shortlist (prec 1 = bf16, 2 = fp8 e4m3)
+struct ShortScanArgs {
+  hipStream_t st;
+  int KS;
+  const void* xfrag;
+  const float* xscale;
+  uint32_t n;
+  const void* qfrag;
+  const float* qscale;
+  uint32_t nq, n_qblocks, seg_rows, n_tasks, n_tiles;
+  uint32_t* tilemin;
+  const float* thr;
+  uint64_t* cand;
+  uint32_t cap;
+  uint32_t* count;
+};
+int short_ks(uint32_t d);
+int short_cand_cap();
+int short_tile_rows();
+hipError_t launch_quantize(hipStream_t st, const float* src, uint32_t ld, uint32_t d, uint32_t r0,
+                           uint32_t r1, uint32_t KS, int prec, void* out, float* scale,
+                           double* norms);
+hipError_t launch_norm_max(hipStream_t st, const double* norms, uint32_t r0, uint32_t r1,
+                           unsigned long long* maxes);
+hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq, int prec, uint32_t d,
+                              const unsigned long long* maxes, float* E);
+hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
+hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a);
+hipError_t launch_select_threshold(hipStream_t st, const uint32_t* tilemin, uint32_t n_tiles, uint32_t nq,
+                                   const uint32_t* ks, uint32_t k, const float* E, float* thr);
+hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t dp, uint32_t d,
+                          const float* queries, uint32_t nq, const uint32_t* ks, uint32_t k,
+                          const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
+                          float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
+                          uint32_t* rescored);
+
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {
   void* dev[8] = {};      // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out)

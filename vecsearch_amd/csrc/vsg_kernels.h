@@ -210,6 +210,38 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
   }
 }
 
+// Merge of the block's per-wave sorted k-lists into one sorted k-list per query, by ranking:
+// a real key's rank in the union is its slot in its own list plus, for every other list, the
+// number of keys below it (binary search). Real keys are unique, so ranks are distinct; keys
+// ranked >= k are dropped and unused slots keep kPadKey.
+// lds: NW x k keys (wave w's list at lds + w * k); out: k keys (LDS or global).
+template <int NW>
+__device__ __forceinline__ void block_rank_merge(const uint64_t* lds, uint32_t k, uint64_t* out,
+                                                 int tid, int nthreads) {
+  for (uint32_t i = tid; i < k; i += nthreads) out[i] = kPadKey;
+  __syncthreads();
+  for (uint32_t i = tid; i < uint32_t(NW) * k; i += nthreads) {
+    const uint32_t w = i / k, s = i - w * k;
+    const uint64_t v = lds[i];
+    if (v == kPadKey) continue;
+    uint32_t rank = s;
+#pragma unroll
+    for (int o = 0; o < NW; ++o) {
+      if (o == int(w)) continue;
+      const uint64_t* l = lds + o * k;
+      uint32_t lo = 0, hi = k;  // first index with l[idx] >= v
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (l[mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      rank += lo;
+      if (rank >= k) break;
+    }
+    if (rank < k) out[rank] = v;
+  }
+}
+
 // Offer keys of the calling wave to a query's k-list; `key` is kPadKey on lanes that carry no
 // candidate. Returns the updated admission threshold (the list's k-th key).
 template <int KP>

@@ -28,15 +28,16 @@
 namespace vsg {
 
 // One pass: queries [q0, q0 + nqp) of the batch (nqp <= QB) against the base. Wave gw scans
-// rows [gw * rpw, min(n, (gw + 1) * rpw)) (rpw a multiple of 64) and writes its k-list of each
-// query of the pass to part[(q * W + gw) * k + slot]. mask (nullable): rows with
-// mask[row] == 0 carry no result. D16 > 0 specialises d = 16 * D16; D16 == 0 handles any d.
+// rows [gw * rpw, min(n, (gw + 1) * rpw)) (rpw a multiple of 64); the block then merges its
+// waves' k-lists and writes one k-list per query of the pass to part[(q * W + block) * k + slot]
+// (W = number of blocks). mask (nullable): rows with mask[row] == 0 carry no result.
+// D16 > 0 specialises d = 16 * D16; D16 == 0 handles any d.
 template <int QB, int R, int D16, int KP, bool L2>
 __global__ __launch_bounds__(kBlock) void scan_exact_kernel(
     const float* __restrict__ tiles, uint32_t n, uint32_t dp, uint32_t d,
     const float* __restrict__ queries, uint32_t q0, uint32_t nqp, uint32_t k, uint32_t rpw,
     uint32_t W, const uint8_t* __restrict__ mask, uint64_t* __restrict__ part) {
-  extern __shared__ float q_lds[];  // QB x dp
+  extern __shared__ float q_lds[];  // QB x dp, then (after the scan) QB x 4 x k merge keys
   const int tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
   const uint32_t gw = blockIdx.x * kWavesPerBlock + (tid >> 6);
 
@@ -79,25 +80,44 @@ __global__ __launch_bounds__(kBlock) void scan_exact_kernel(
       }
   }
 
+  // block merge: the wave lists go to LDS (reusing the query area), one list per query leaves
+  __syncthreads();
+  uint64_t* keys = reinterpret_cast<uint64_t*>(q_lds);
+  const int wave = tid >> 6;
 #pragma unroll
   for (int b = 0; b < QB; ++b) {
     if (uint32_t(b) >= nqp) continue;
-    uint64_t* dst = part + (size_t(q0 + b) * W + gw) * k;
 #pragma unroll
     for (int r = 0; r < KP / 64; ++r) {
       const uint32_t s = r * 64 + lane;
-      if (s < k) dst[s] = lst[b].e[r];
+      if (s < k) keys[(b * kWavesPerBlock + wave) * k + s] = lst[b].e[r];
     }
   }
+  __syncthreads();
+  for (uint32_t b = 0; b < nqp; ++b)
+    block_rank_merge<kWavesPerBlock>(keys + b * kWavesPerBlock * k, k,
+                                     part + (size_t(q0 + b) * W + blockIdx.x) * k, tid, kBlock);
 }
 
 // ---- merge -------------------------------------------------------------------------------
-// One block of 256 threads per query over that query's sorted lists: k rounds of a
-// block-wide minimum over the list heads. Keys of real rows are unique (a row is scanned by
-// one wave of a query), so exactly one thread owns the winner and advances that list.
+// One block of 256 threads per query reduces that query's sorted lists to its k smallest keys.
 // Lists of query q: part + (list_off ? list_off[q] : q * W) * L, count list_off[q+1] -
-// list_off[q] (or W), L entries each. Query q returns ks ? ks[q] : k results into row q of
-// the outputs (stride ldo), padding the rest of the row with (kNone, +inf).
+// list_off[q] (or W), L >= k entries each. Query q returns kq = (ks ? ks[q] : k) results into
+// row q of the outputs (stride ldo), padding the rest of the row with (kNone, +inf).
+//
+// 1. Threshold: m = ceil(kq / #lists) leading keys of every list form a sample of >= kq keys;
+//    its kq-th smallest key T is >= the answer's kq-th key (a subset's kq-th smallest is never
+//    below the whole set's), and because the lists are sorted and cover disjoint rows, T is
+//    close to it. The sample is ranked in LDS (by counting up to 1024 keys, else by a bitonic
+//    sort of up to kGather keys).
+// 2. Every thread walks its lists from the head while keys are <= T and appends them to LDS.
+// 3. The gathered keys (about kq of them) are ranked: by counting (<= 256 keys) or by a
+//    bitonic sort (<= kGather keys). Keys of real rows are unique (a row is scanned once per
+//    query), so ranks are distinct.
+// 4. A sample or gather larger than kGather keys (tiny lists, huge list counts, or many exact
+//    ties) falls back to kq rounds of a block-wide minimum over the list heads.
+constexpr int kGather = 4096;
+
 __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -109,15 +129,52 @@ __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
   return v;
 }
 
+__device__ __forceinline__ uint64_t block_min64(uint64_t v, uint64_t* red) {
+  const uint64_t wm = wave_min64(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+  __syncthreads();
+  uint64_t m = red[0];
+#pragma unroll
+  for (int i = 1; i < 4; ++i) m = red[i] < m ? red[i] : m;
+  __syncthreads();
+  return m;
+}
+
+// Ascending bitonic sort of buf[0, n2) (n2 a power of two) by a 256-thread block.
+__device__ void bitonic_sort(uint64_t* buf, uint32_t n2, uint32_t tid) {
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = tid; i < n2 / 2; i += 256) {
+        const uint32_t lo = 2 * i - (i & (stride - 1));
+        const uint32_t hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = buf[lo], b = buf[hi];
+        if ((a > b) == up) {
+          buf[lo] = b;
+          buf[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ void put_key(uint32_t* oi, float* od, uint32_t r, uint64_t key) {
+  oi[r] = key == kPadKey ? kNoneId : key_id(key);
+  od[r] = key_dist(key);
+}
+
 __global__ __launch_bounds__(256) void merge_lists_kernel(
     const uint64_t* __restrict__ part, const uint64_t* __restrict__ list_off, uint32_t W,
     uint32_t L, const uint32_t* __restrict__ ks, uint32_t k, const uint32_t* __restrict__ qmap,
     uint32_t* __restrict__ out_ids, float* __restrict__ out_dists, uint32_t ldo) {
   __shared__ uint64_t red[4];
+  __shared__ uint64_t buf[kGather];
+  __shared__ uint32_t cnt;
   const uint32_t q = qmap ? qmap[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
   const uint64_t l0 = list_off ? list_off[q] : uint64_t(q) * W;
   const uint32_t nl = list_off ? uint32_t(list_off[q + 1] - l0) : W;
-  const uint32_t kq = min(ks ? ks[q] : k, ldo);
+  const uint32_t kq = min(min(ks ? ks[q] : k, ldo), L);
   const uint64_t* lists = part + l0 * L;
   uint32_t* oi = out_ids + size_t(q) * ldo;
   float* od = out_dists + size_t(q) * ldo;
@@ -125,8 +182,81 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
     oi[c] = kNoneId;
     od[c] = __builtin_inff();
   }
+  if (kq == 0 || nl == 0) {
+    for (uint32_t c = tid; c < kq; c += 256) put_key(oi, od, c, kPadKey);
+    return;
+  }
 
-  // heads of this thread's lists stay in registers; advancing a list loads one new key
+  // 1. threshold from a sample of every list's leading keys
+  const uint32_t m = (kq + nl - 1) / nl;
+  const uint32_t ns = nl * m;
+  uint64_t T = kPadKey;
+  if (ns <= uint32_t(kGather)) {
+    for (uint32_t i = tid; i < ns; i += 256) {
+      const uint32_t l = i / m, p = i - l * m;
+      buf[i] = lists[size_t(l) * L + p];
+    }
+    __syncthreads();
+    if (ns <= 1024) {
+      for (uint32_t i = tid; i < ns; i += 256) {
+        const uint64_t v = buf[i];
+        uint32_t below = 0, equal_before = 0;
+        for (uint32_t j = 0; j < ns; ++j) {
+          const uint64_t u = buf[j];
+          below += u < v;
+          equal_before += (u == v) & (j < i);
+        }
+        if (below + equal_before == kq - 1) red[0] = v;  // the unique key of rank kq - 1
+      }
+      __syncthreads();
+      T = red[0];
+    } else {
+      uint32_t n2 = 2048;
+      while (n2 < ns) n2 <<= 1;
+      for (uint32_t i = ns + tid; i < n2; i += 256) buf[i] = kPadKey;
+      __syncthreads();
+      bitonic_sort(buf, n2, tid);
+      T = buf[kq - 1];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) cnt = 0;
+  __syncthreads();
+
+  // 2. gather real keys <= T
+  for (uint32_t l = tid; l < nl; l += 256) {
+    const uint64_t* lp = lists + size_t(l) * L;
+    for (uint32_t p = 0; p < L; ++p) {
+      const uint64_t v = lp[p];
+      if (v > T || v == kPadKey) break;
+      const uint32_t slot = atomicAdd(&cnt, 1u);
+      if (slot < uint32_t(kGather)) buf[slot] = v;
+    }
+  }
+  __syncthreads();
+  const uint32_t c = cnt;
+
+  if (c <= uint32_t(kGather)) {
+    if (c <= 256) {  // 3a. rank by counting
+      if (tid < c) {
+        const uint64_t v = buf[tid];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < c; ++j) rank += buf[j] < v;
+        if (rank < kq) put_key(oi, od, rank, v);
+      }
+    } else {  // 3b. bitonic sort of the next power of two
+      uint32_t n2 = 512;
+      while (n2 < c) n2 <<= 1;
+      for (uint32_t i = c + tid; i < n2; i += 256) buf[i] = kPadKey;
+      __syncthreads();
+      bitonic_sort(buf, n2, tid);
+      for (uint32_t r = tid; r < kq && r < c; r += 256) put_key(oi, od, r, buf[r]);
+    }
+    for (uint32_t r = c + tid; r < kq; r += 256) put_key(oi, od, r, kPadKey);
+    return;
+  }
+
+  // 4. fallback: k rounds over the list heads (heads cached in registers)
   uint32_t pos[kMergeHeads];
   uint64_t head[kMergeHeads];
   uint64_t best = kPadKey;
@@ -135,31 +265,19 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
   for (int h = 0; h < kMergeHeads; ++h) {
     pos[h] = 0;
     const uint32_t l = tid + h * 256;
-    head[h] = (l < nl && L > 0) ? lists[size_t(l) * L] : kPadKey;
+    head[h] = l < nl ? lists[size_t(l) * L] : kPadKey;
     if (head[h] < best) {
       best = head[h];
       best_h = h;
     }
   }
   for (uint32_t r = 0; r < kq; ++r) {
-    const uint64_t wm = wave_min64(best);
-    if ((tid & 63) == 0) red[tid >> 6] = wm;
-    __syncthreads();
-    uint64_t m = red[0];
-#pragma unroll
-    for (int i = 1; i < 4; ++i) m = red[i] < m ? red[i] : m;
-    __syncthreads();
+    const uint64_t m = block_min64(best, red);
     if (m == kPadKey) {
-      for (uint32_t rr = r + tid; rr < kq; rr += 256) {
-        oi[rr] = kNoneId;
-        od[rr] = __builtin_inff();
-      }
+      for (uint32_t rr = r + tid; rr < kq; rr += 256) put_key(oi, od, rr, kPadKey);
       return;
     }
-    if (tid == 0) {
-      oi[r] = key_id(m);
-      od[r] = key_dist(m);
-    }
+    if (tid == 0) put_key(oi, od, r, m);
     if (best == m) {  // this thread owns the winner: advance that list
       best = kPadKey;
       int nh = -1;
@@ -246,7 +364,7 @@ hipError_t launch_scan(int qb, bool l2, uint32_t k, uint32_t n_blocks, hipStream
                        uint64_t* part) {
   ScanArgs a;
   a.grid = dim3(n_blocks);
-  a.lds = size_t(qb) * dp * sizeof(float);
+  a.lds = std::max(size_t(qb) * dp * sizeof(float), size_t(qb) * kWavesPerBlock * k * sizeof(uint64_t));
   a.st = st;
   a.tiles = tiles;
   a.n = n;
@@ -257,7 +375,7 @@ hipError_t launch_scan(int qb, bool l2, uint32_t k, uint32_t n_blocks, hipStream
   a.nqp = nqp;
   a.k = k;
   a.rpw = rpw;
-  a.W = n_blocks * kWavesPerBlock;
+  a.W = n_blocks;
   a.mask = mask;
   a.part = part;
   const int kp = k <= 64 ? 64 : 128;
