@@ -119,7 +119,7 @@ struct vsg_index {
 
   // shortlist workspace
   DevBuf<uint8_t> s_qfrag;
-  DevBuf<float> s_qscale, s_E, s_thr;
+  DevBuf<float> s_qscale, s_E, s_thr, s_qaux;
   DevBuf<double> s_qnorms;
   DevBuf<uint32_t> s_tilemin, s_count, s_flags, s_rescored, s_nover;
   DevBuf<uint64_t> s_cand;
@@ -166,6 +166,7 @@ struct vsg_index {
     s_qscale.release();
     s_E.release();
     s_thr.release();
+    s_qaux.release();
     s_qnorms.release();
     s_tilemin.release();
     s_count.release();
@@ -374,17 +375,19 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
   const auto& c = h->prec[p];
   const int ks = h->ks;
   const uint32_t cap = uint32_t(vsg::short_cand_cap());
-  const uint32_t tile_rows = uint32_t(vsg::short_tile_rows());
+  const uint32_t tile_rows = uint32_t(vsg::short_tile_rows_for(p, ks));
   const uint32_t n_tiles = uint32_t((h->n + tile_rows - 1) / tile_rows);
   const size_t round = std::min<size_t>(nq, 4096);
   const size_t chunk = p == 1 ? 1024 : 512;
-  const size_t qpad = (round + 63) / 64 * 64;
+  const size_t grp = size_t(vsg::short_group(p, ks)) * 16;  // queries per stage of the MFMA scan
+  const size_t qpad = (round + grp - 1) / grp * grp;
   hipError_t e;
   if ((e = h->s_qfrag.reserve(qpad / 16 * ks * chunk)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_qnorms.reserve(qpad * 3)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_qaux.reserve(qpad / 16 * 32 + vsg::short_aux_pad())) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_tilemin.reserve(round * n_tiles)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
@@ -414,30 +417,33 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     a.xscale = c.scale;
     a.n = uint32_t(h->n);
     a.qfrag = h->s_qfrag.p;
-    a.qscale = h->s_qscale.p;
+    a.qaux = h->s_qaux.p;
     a.nq = rq;
-    a.n_qblocks = (rq + 63) / 64;
-    // about 16 wave tasks per CU, segments a multiple of the tile
-    const size_t want_seg = std::max<size_t>(1, size_t(h->num_cus) * 16 / a.n_qblocks);
-    size_t seg_rows = (h->n + want_seg - 1) / want_seg;
-    seg_rows = std::max<size_t>(tile_rows, (seg_rows + tile_rows - 1) / tile_rows * tile_rows);
-    a.seg_rows = uint32_t(seg_rows);
-    a.n_tasks = uint32_t(a.n_qblocks * ((h->n + seg_rows - 1) / seg_rows));
-    a.n_tiles = n_tiles;
+    // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk)
+    // (a looser bound only grows the candidate set, checked against its capacity)
+    const uint32_t tstride = uint32_t(std::min<size_t>(16, std::max<size_t>(1, n_tiles / (4 * k))));
+    const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
     a.tilemin = h->s_tilemin.p;
-    a.thr = h->s_thr.p;
     a.cand = h->s_cand.p;
     a.cap = cap;
     a.count = h->s_count.p;
+    VSG_HIP(vsg::launch_query_aux(st, nullptr, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
+                                  h->s_qaux.p));
     {
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
+      a.tstride = tstride;
+      a.n_space = a.n_blocks = space1;
       VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
       ts.launches++;
     }
-    VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, n_tiles, rq, nullptr, uint32_t(k), h->s_E.p,
+    VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, space1, rq, nullptr, uint32_t(k), h->s_E.p,
                                          h->s_thr.p));
+    VSG_HIP(vsg::launch_query_aux(st, h->s_thr.p, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
+                                  h->s_qaux.p));
     {
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
+      a.tstride = 1;
+      a.n_space = a.n_blocks = n_tiles;
       VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
       ts.launches++;
     }

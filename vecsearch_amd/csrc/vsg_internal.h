@@ -34,17 +34,23 @@ struct ShortScanArgs {
   const float* xscale;
   uint32_t n;
   const void* qfrag;
-  const float* qscale;
-  uint32_t nq, n_qblocks, seg_rows, n_tasks, n_tiles;
+  const float* qaux;  // per query fragment: 16 thresholds, then 16 scales (query_aux_kernel)
+  uint32_t nq;
+  uint32_t n_blocks;  // one block per visited tile
+  uint32_t n_space;   // visited tiles: pass 1 ceil(n_tiles / tstride), pass 2 n_tiles
+  uint32_t tstride;   // pass 1 visits every tstride-th tile; pass 2 uses 1
   uint32_t* tilemin;
-  const float* thr;
   uint64_t* cand;
   uint32_t cap;
   uint32_t* count;
 };
 int short_ks(uint32_t d);
 int short_cand_cap();
-int short_tile_rows();
+int short_tile_rows_for(int prec, int ks);
+int short_group(int prec, int ks);
+int short_aux_pad();
+hipError_t launch_query_aux(hipStream_t st, const float* thr, const float* qscale, uint32_t nq,
+                            uint32_t nfrag_pad, float* aux);
 hipError_t launch_quantize(hipStream_t st, const float* src, uint32_t ld, uint32_t d, uint32_t r0,
                            uint32_t r1, uint32_t KS, int prec, void* out, float* scale,
                            double* norms);

@@ -9,7 +9,7 @@
 // C = {x : dh(x) <= Dk + 2E} contains the answer, and ranking C by exact (d, id) returns it.
 //
 // How C is found without a per-query top-k inside the GEMM:
-//   pass 1  tile minima: for every query and every tile of kTileRows rows, the smallest dh;
+//   pass 1  tile minima: for every query and every tile (512 or 256 rows), the smallest dh;
 //   select  U = the k-th smallest tile minimum (k distinct rows have dh <= U, so Dk <= U);
 //   pass 2  emit every row with dh <= U + 2E into the query's candidate buffer;
 //   rescore Dk is the k-th smallest dh among the emitted rows (they include every row with
@@ -46,7 +46,7 @@ namespace vsg {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
 typedef __attribute__((ext_vector_type(4))) float f32x4m_t;
 
-constexpr int kTileRows = 256;  // rows per tile minimum (pass 1)
+// rows per tile of the MFMA scan: 8 waves x RF row fragments x 16 rows (RF = 4 or 2)
 
 // ---- conversion into fragment layouts ----------------------------------------------------
 
@@ -150,10 +150,13 @@ __global__ void query_bound_kernel(const double* __restrict__ qnorms, uint32_t n
   E[q] = float(e * (1.0 + 1e-6) + 1e-12);
 }
 
-// ---- the MFMA scan -----------------------------------------------------------------------
-// A wave owns a task (64 queries = 4 query fragments of 16, one row segment). PASS 1 writes
-// tile minima; PASS 2 emits candidates. KS: 32-dim k-steps (template: fully unrolled, the A
-// operands of all k-steps live in registers).
+// ---- the MFMA scan (rows stationary, queries streaming) ---------------------------------
+// A block of 8 waves owns one tile of 8 * RF * 16 base rows: wave w keeps rows w*RF*16 .. +RF*16
+// of the tile as B operands of every k-step in registers (loaded once from HBM). The batch's
+// query fragments (16 queries x all k-steps, small and L2-resident) stream past them through
+// a double-buffered LDS ring, kGroup fragments per stage. PASS 1 writes, per query, the tile's
+// smallest dh (tilemin[q * n_space + j], tile j of the visited tiles); PASS 2 emits
+// candidates with dh <= thr[q]. Pass 1 visits every tstride-th tile.
 template <int PREC>
 struct Frag;
 template <>
@@ -178,94 +181,222 @@ __device__ __forceinline__ f32x4m_t mfma<2>(long a, long b, f32x4m_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
 }
 
-constexpr int kQF = 4;  // query fragments per wave (64 queries)
+constexpr int kWaves = 8;   // waves per block
+constexpr int kNBuf = 4;    // LDS stages in the ring
+constexpr int kAhead = kNBuf - 1;  // stages in flight ahead of the one being consumed
+
+// Query fragments per LDS stage: the stage's 1 KB pieces must split evenly over the 8 waves
+// (every wave then issues the same number of loads per stage, which the vmcnt wait counts).
+__host__ __device__ constexpr int stage_group(int prec, int ks) {
+  return (ks * (prec == 1 ? 1024 : 512) / 1024) % 8 == 0 ? 1
+         : (ks * (prec == 1 ? 1024 : 512) / 1024) % 4 == 0 ? 2
+         : (ks * (prec == 1 ? 1024 : 512) / 1024) % 2 == 0 ? 4
+                                                            : 8;
+}
+
+// row fragments (16 rows each) per wave: the B operands of all k-steps stay in registers
+__host__ __device__ constexpr int rows_frags(int prec, int ks) { return ks * (prec == 1 ? 4 : 2) <= 48 ? 4 : 2; }
+
+// Minimum over each group of 16 lanes (a DPP row), left in every lane of the row: two
+// quad_perm swaps, then rotations by 4 and 8 within the row.
+__device__ __forceinline__ float row16_min(float v) {
+  v = fminf(v, dpp_f<kDppXor1>(v));
+  v = fminf(v, dpp_f<kDppXor2>(v));
+  v = fminf(v, dpp_f<0x124>(v));  // row_ror:4
+  v = fminf(v, dpp_f<0x128>(v));  // row_ror:8
+  return v;
+}
+
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef __attribute__((address_space(1))) void* glb_vptr;
+
+// 16 bytes per lane from global straight into LDS at lds_base + 16 * lane (global_load_lds_dwordx4);
+// lds_base must be the same for the whole wave.
+__device__ __forceinline__ void load_lds16(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 16, 0, 0);
+}
+__device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 4, 0, 0);
+}
+// s_waitcnt vmcnt(N), N < 64 (gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt
+// left at their maxima so only vector-memory operations are waited for)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int I>
+struct Slot {
+  static constexpr int value = I;
+};
+
+template <int PREC, int KS>
+struct StageShape {
+  static constexpr int RF = rows_frags(PREC, KS);
+  static constexpr int G = stage_group(PREC, KS);  // query fragments per stage
+  static constexpr int CB = Frag<PREC>::kChunkBytes;
+  static constexpr int FRAG_BYTES = KS * CB;       // one query fragment, all k-steps
+  static constexpr int STAGE_Q = G * FRAG_BYTES;   // query operands of a stage
+  // per query of the stage: threshold, scale (G x 32 floats), rounded up to whole 256-byte
+  // wave loads (64 lanes x 4 bytes); the caller pads qaux by kAuxPad floats past the last stage
+  static constexpr int AUX_BYTES = (G * 32 * 4 + 255) / 256 * 256;
+  static constexpr int STAGE_BYTES = STAGE_Q + AUX_BYTES;
+  static constexpr int PIECES = STAGE_Q / 1024;    // 1 KB pieces (one wave load each)
+  static constexpr int PER_WAVE = PIECES / kWaves; // loads per wave per stage (exact)
+  static constexpr int AUX_LOADS = AUX_BYTES / 256;
+  static constexpr int LOADS = PER_WAVE + AUX_LOADS;  // vector-memory loads per thread per stage
+  static_assert(PIECES % kWaves == 0, "stage pieces must split evenly over the waves");
+};
 
 template <int PREC, int KS, int PASS>
-__global__ __launch_bounds__(256) void mfma_scan_kernel(
+__global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
-    const void* __restrict__ qfrag, const float* __restrict__ qscale, uint32_t nq,
-    uint32_t n_qblocks, uint32_t seg_rows, uint32_t n_tasks, uint32_t n_tiles,
-    uint32_t* __restrict__ tilemin, const float* __restrict__ thr, uint64_t* __restrict__ cand,
-    uint32_t cap, uint32_t* __restrict__ count) {
+    const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t tstride,
+    uint32_t n_space, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ cand, uint32_t cap,
+    uint32_t* __restrict__ count) {
   typedef typename Frag<PREC>::T FT;
-  const int lane = threadIdx.x & 63;
-  const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (task >= n_tasks) return;
-  const uint32_t qb = task % n_qblocks, seg = task / n_qblocks;
-  const uint32_t row0 = seg * seg_rows, row1 = min(n, row0 + seg_rows);
-  if (row0 >= row1) return;
+  typedef StageShape<PREC, KS> SH;
+  constexpr int RF = SH::RF, G = SH::G, LOADS = SH::LOADS;
+  // Four ring slots as distinct LDS objects: the compiler can then tell the slot being read
+  // from the slots still being filled by LDS-DMA loads, and waits only on the counted vmcnt.
+  __shared__ __attribute__((aligned(16))) uint8_t slot0[SH::STAGE_BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t slot1[SH::STAGE_BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t slot2[SH::STAGE_BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t slot3[SH::STAGE_BYTES];
+  __shared__ float red[2][kWaves * G * 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t j = blockIdx.x;  // visited tile
+  const uint32_t tile = j * tstride;
+  const uint32_t nrb = (n + 15) >> 4;
+  const uint32_t rb0 = tile * (kWaves * RF) + wave * RF;  // first row block of this wave
+
+  // this wave's rows, all k-steps (zero for row blocks past the base)
+  FT b[RF][KS];
   const FT* xf = reinterpret_cast<const FT*>(xfrag);
-  const FT* qf = reinterpret_cast<const FT*>(qfrag);
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[f][s] = (rb0 + f < nrb) ? xf[(size_t(rb0 + f) * KS + s) * 64 + lane] : FT{};
+  float xs[RF];
+  bool ok[RF];
+#pragma unroll
+  for (int f = 0; f < RF; ++f) {
+    const uint32_t row = (rb0 + f) * 16 + (lane & 15);
+    ok[f] = row < n;
+    xs[f] = (PREC == 2 && ok[f]) ? xscale[row] : 1.f;
+  }
 
-  // A operands: 4 query fragments (queries qb*64 + f*16 + ..) for every k-step
-  FT a[kQF][KS];
+  // The caller pads the query fragments (and qaux) to whole stages, so every stage load is
+  // in bounds and every wave issues exactly LOADS loads per stage.
+  const uint32_t nstages = (((nq + 15) >> 4) + G - 1) / G;
+  const uint8_t* qsrc = static_cast<const uint8_t*>(qfrag);
+  auto slot = [&](auto I) -> uint8_t* {
+    constexpr int v = decltype(I)::value & 3;
+    return v == 0 ? slot0 : v == 1 ? slot1 : v == 2 ? slot2 : slot3;
+  };
+  auto fetch = [&](uint8_t* dst, uint32_t st) {
 #pragma unroll
-  for (int f = 0; f < kQF; ++f)
-#pragma unroll
-    for (int s = 0; s < KS; ++s) a[f][s] = qf[(size_t(qb * kQF + f) * KS + s) * 64 + lane];
-
-  // per-lane query indices: value i of fragment f is query qb*64 + f*16 + 4*(lane/16) + i
-  const uint32_t qbase = qb * 64 + 4 * (lane >> 4);
-  float qs[kQF][4], th[kQF][4], run[kQF][4];
-#pragma unroll
-  for (int f = 0; f < kQF; ++f)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t q = qbase + f * 16 + i;
-      qs[f][i] = (PREC == 2 && q < nq) ? qscale[q] : 1.f;
-      th[f][i] = (PASS == 2 && q < nq) ? thr[q] : -__builtin_inff();
-      run[f][i] = __builtin_inff();
+    for (int r = 0; r < SH::PER_WAVE; ++r) {
+      const int piece = wave + r * kWaves;
+      load_lds16(qsrc + size_t(st) * SH::STAGE_Q + size_t(piece) * 1024 + lane * 16, dst + piece * 1024);
     }
+#pragma unroll
+    for (int r = 0; r < SH::AUX_LOADS; ++r)
+      load_lds4(qaux + size_t(st) * G * 32 + r * 64 + lane, dst + SH::STAGE_Q + r * 256);
+  };
+  auto flush_min = [&](uint32_t st) {  // tile minima of stage st (pass 1), after a barrier
+    if (threadIdx.x < G * 16) {
+      const float* rp = red[st & 1];
+      const uint32_t q = st * G * 16 + threadIdx.x;
+      float m = rp[threadIdx.x];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w) m = fminf(m, rp[w * (G * 16) + threadIdx.x]);
+      if (q < nq) tilemin[size_t(q) * n_space + j] = ord_bits(m);
+    }
+  };
 
-  for (uint32_t r = row0; r < row1; r += 16) {
-    const uint32_t rb = r >> 4;
-    FT b[KS];
+  // consume stage st from slot I; refill slot I + 3 (consumed at st - 1) with stage st + 3
+  auto step = [&](auto I, uint32_t st) {
+    const uint32_t later = min(nstages - 1 - st, uint32_t(kAhead - 1));
+    if (later >= 2) wait_vm<2 * LOADS>();
+    else if (later == 1) wait_vm<LOADS>();
+    else wait_vm<0>();
+    __syncthreads();  // every wave's part of stage st is in LDS; stage st - 1 is consumed
+    if (PASS == 1 && st > 0) flush_min(st - 1);
+    if (st + kAhead < nstages) fetch(slot(Slot<decltype(I)::value + kAhead>()), st + kAhead);
+    const uint8_t* stage = slot(I);
+    const FT* as = reinterpret_cast<const FT*>(stage);
+    const float* aux = reinterpret_cast<const float*>(stage + SH::STAGE_Q);
+    float* rd = red[st & 1];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) b[s] = xf[(size_t(rb) * KS + s) * 64 + lane];
-    f32x4m_t acc[kQF];
+    for (int g = 0; g < G; ++g) {
+      const uint32_t qf = st * G + g;
+      f32x4m_t acc[RF];
 #pragma unroll
-    for (int f = 0; f < kQF; ++f) acc[f] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
+      for (int f = 0; f < RF; ++f) acc[f] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
+      for (int s = 0; s < KS; ++s) {
+        const FT a = as[(g * KS + s) * 64 + lane];
 #pragma unroll
-      for (int f = 0; f < kQF; ++f) acc[f] = mfma<PREC>(a[f][s], b[s], acc[f]);
-
-    const uint32_t row = r + (lane & 15);
-    const bool ok = row < row1;
-    const float xs = PREC == 2 ? (ok ? xscale[row] : 1.f) : 1.f;
-#pragma unroll
-    for (int f = 0; f < kQF; ++f)
+        for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(a, b[f][s], acc[f]);
+      }
+      // value i of acc[f] belongs to query qf*16 + 4*(lane/16) + i and row (rb0+f)*16 + lane%16
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float s = PREC == 2 ? acc[f][i] * qs[f][i] * xs : acc[f][i];
-        const float dh = ok ? 1.0f - s : __builtin_inff();
+        const int qi = 4 * (lane >> 4) + i;  // query within the fragment
+        const uint32_t q = qf * 16 + qi;
+        const float qsc = PREC == 2 ? aux[g * 32 + 16 + qi] : 1.f;
         if (PASS == 1) {
-          run[f][i] = fminf(run[f][i], dh);
-        } else if (dh <= th[f][i]) {
-          const uint32_t q = qbase + f * 16 + i;
-          const uint32_t slot = atomicAdd(count + q, 1u);
-          if (slot < cap) cand[size_t(q) * cap + slot] = make_key(dh, row);
+          float m = __builtin_inff();
+#pragma unroll
+          for (int f = 0; f < RF; ++f) {
+            const float sc = PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i];
+            m = fminf(m, ok[f] ? 1.0f - sc : __builtin_inff());
+          }
+          m = row16_min(m);
+          if ((lane & 15) == 0) rd[wave * (G * 16) + g * 16 + qi] = m;
+        } else if (q < nq) {  // padding queries of the last stage never emit
+          const float t = aux[g * 32 + qi];
+#pragma unroll
+          for (int f = 0; f < RF; ++f) {
+            const float sc = PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i];
+            const float dh = 1.0f - sc;
+            if (ok[f] && dh <= t) {
+              const uint32_t row = (rb0 + f) * 16 + (lane & 15);
+              const uint32_t slot_i = atomicAdd(count + q, 1u);
+              if (slot_i < cap) cand[size_t(q) * cap + slot_i] = make_key(dh, row);
+            }
+          }
         }
       }
-
-    if (PASS == 1 && ((r + 16 - row0) % kTileRows == 0 || r + 16 >= row1)) {
-      // tile done: minimum over the 16 row lanes of each lane group, one writer per group
-      const uint32_t tile = r / kTileRows;
-#pragma unroll
-      for (int f = 0; f < kQF; ++f)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          float v = run[f][i];
-          v = fminf(v, __shfl_xor(v, 1, 64));
-          v = fminf(v, __shfl_xor(v, 2, 64));
-          v = fminf(v, __shfl_xor(v, 4, 64));
-          v = fminf(v, __shfl_xor(v, 8, 64));
-          const uint32_t q = qbase + f * 16 + i;
-          if ((lane & 15) == 0 && q < nq) tilemin[size_t(q) * n_tiles + tile] = ord_bits(v);
-          run[f][i] = __builtin_inff();
-        }
     }
+  };
+
+  if (nstages > 0) fetch(slot0, 0);
+  if (nstages > 1) fetch(slot1, 1);
+  if (nstages > 2) fetch(slot2, 2);
+  for (uint32_t st0 = 0; st0 < nstages; st0 += 4) {
+    step(Slot<0>(), st0);
+    if (st0 + 1 < nstages) step(Slot<1>(), st0 + 1);
+    if (st0 + 2 < nstages) step(Slot<2>(), st0 + 2);
+    if (st0 + 3 < nstages) step(Slot<3>(), st0 + 3);
   }
+  if (PASS == 1 && nstages > 0) {  // tile minima of the last stage
+    __syncthreads();
+    flush_min(nstages - 1);
+  }
+}
+
+// Builds the per-fragment auxiliary block of the query stages: for every fragment f,
+// aux[f * 32 + i] = thr[q] and aux[f * 32 + 16 + i] = qscale[q] (q = 16 f + i; padding -inf / 1).
+__global__ void query_aux_kernel(const float* __restrict__ thr, const float* __restrict__ qscale,
+                                 uint32_t nq, uint32_t nfrag_pad, float* __restrict__ aux) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nfrag_pad * 16) return;
+  const uint32_t f = t >> 4, i = t & 15, q = t;
+  aux[f * 32 + i] = (thr && q < nq) ? thr[q] : -__builtin_inff();
+  aux[f * 32 + 16 + i] = (qscale && q < nq) ? qscale[q] : 1.f;
 }
 
 // ---- threshold select --------------------------------------------------------------------
@@ -438,6 +569,18 @@ hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq,
   return hipGetLastError();
 }
 
+// query fragments are staged G at a time: callers pad fragment and aux buffers to whole stages,
+// and the aux buffer by short_aux_pad() more floats (a stage reads whole 256-byte loads)
+int short_group(int prec, int ks) { return stage_group(prec, ks); }
+int short_aux_pad() { return 64; }
+
+hipError_t launch_query_aux(hipStream_t st, const float* thr, const float* qscale, uint32_t nq,
+                            uint32_t nfrag_pad, float* aux) {
+  hipLaunchKernelGGL(query_aux_kernel, dim3((nfrag_pad * 16 + 255) / 256), dim3(256), 0, st, thr, qscale, nq,
+                     nfrag_pad, aux);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(unsigned(std::min<size_t>(1024, (n + 255) / 256))), dim3(256), 0,
@@ -448,15 +591,14 @@ hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
 namespace {
 template <int PREC, int KS>
 hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
-  const dim3 grid((a.n_tasks + 3) / 4);
+  const size_t lds = 0;  // static: four ring slots and the reduction buffer
+  const dim3 grid(a.n_blocks), block(64 * kWaves);
   if (pass == 1)
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, dim3(256), 0, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qscale, a.nq, a.n_qblocks, a.seg_rows, a.n_tasks, a.n_tiles, a.tilemin,
-                       a.thr, a.cand, a.cap, a.count);
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.cand, a.cap, a.count);
   else
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, dim3(256), 0, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qscale, a.nq, a.n_qblocks, a.seg_rows, a.n_tasks, a.n_tiles, a.tilemin,
-                       a.thr, a.cand, a.cap, a.count);
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.cand, a.cap, a.count);
   return hipGetLastError();
 }
 
@@ -467,6 +609,7 @@ hipError_t launch_mfma_p(int pass, const ShortScanArgs& a) {
     case 4: return launch_mfma_ks<PREC, 4>(pass, a);
     case 8: return launch_mfma_ks<PREC, 8>(pass, a);
     case 12: return launch_mfma_ks<PREC, 12>(pass, a);
+    case 24: return launch_mfma_ks<PREC, 24>(pass, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -478,8 +621,12 @@ int short_ks(uint32_t d) {
   if (ks <= 4) return 4;
   if (ks <= 8) return 8;
   if (ks <= 12) return 12;
-  return 0;  // too wide for the register-resident query operands
+  if (ks <= 24) return 24;
+  return 0;  // too wide for the register-resident row operands
 }
+
+// rows per tile of the MFMA scan for this precision and width (8 waves x row fragments x 16)
+int short_tile_rows_for(int prec, int ks) { return kWaves * rows_frags(prec, ks) * 16; }
 
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a) {
   return prec == 1 ? launch_mfma_p<1>(pass, a) : launch_mfma_p<2>(pass, a);
@@ -514,6 +661,6 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 }
 
 int short_cand_cap() { return kCandCap; }
-int short_tile_rows() { return kTileRows; }
+
 
 }  // namespace vsg
