@@ -220,6 +220,19 @@ def test_shortlist_other_dims(d):
         assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
 
 
+@pytest.mark.parametrize("scale", [1e-3, 40.0])
+def test_shortlist_unnormalized_ip(scale):
+    # inner products far from 1 in magnitude: the thresholds and score cuts must stay
+    # conservative where a float ulp is much larger (or smaller) than the fixed cushions
+    rng = np.random.default_rng(48)
+    x = (dg.mixture(66_000, 128, seed=48) * rng.uniform(0.5, 2.0, (66_000, 1))).astype(np.float32) * scale
+    q = (dg.mixture(80, 128, seed=49) * scale).astype(np.float32)
+    gpu = vs.GpuBruteForceIndex(128, "ip")
+    gpu.add(x)
+    for method in ("bf16", "fp8"):
+        assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
+
+
 def test_shortlist_overflow_falls_back_exactly():
     # 3,000 identical rows tie with each other: the candidate set exceeds the buffer and the
     # queries near them must be answered by the exact scan, with ties to the lower id.

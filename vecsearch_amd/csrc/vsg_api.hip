@@ -123,6 +123,8 @@ struct vsg_index {
   DevBuf<double> s_qnorms;
   DevBuf<uint32_t> s_tilemin, s_count, s_flags, s_rescored, s_nover;
   DevBuf<uint64_t> s_cand;
+  DevBuf<uint64_t> s_emit;
+  DevBuf<uint32_t> s_emit_count, s_emit_over;
   HostBuf<uint32_t> s_hover;
   uint64_t st_emitted = 0, st_rescored = 0, st_fallback = 0;
   size_t st_nq = 0;
@@ -174,6 +176,9 @@ struct vsg_index {
     s_rescored.release();
     s_nover.release();
     s_cand.release();
+    s_emit.release();
+    s_emit_count.release();
+    s_emit_over.release();
     s_hover.release();
     q.release();
     part.release();
@@ -394,7 +399,15 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
   if ((e = h->s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = h->s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_hover.reserve(1)) != hipSuccess) return fail_hip("pinned staging", e);
+  // pass-2 emission regions, one per wave (64 rows): a round of 4,096 queries that each keep a
+  // few hundred candidates out of n rows fills a region to a few dozen entries on average;
+  // 1,024 leaves a wide margin for clustered data (an overflow sends the round to the exact scan)
+  const uint32_t wave_cap = 1024;
+  const size_t n_regions = size_t(n_tiles) * vsg::short_waves();
+  if ((e = h->s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_emit_over.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = h->s_hover.reserve(2)) != hipSuccess) return fail_hip("pinned staging", e);
   h->st_emitted = h->st_rescored = h->st_fallback = 0;
   h->st_nq = 0;
 
@@ -409,6 +422,7 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     VSG_HIP(vsg::launch_query_bound(st, h->s_qnorms.p, rq, p, uint32_t(32 * ks), c.maxes, h->s_E.p));
     VSG_HIP(vsg::launch_zero_u32(st, h->s_count.p, rq));
     VSG_HIP(vsg::launch_zero_u32(st, h->s_nover.p, 1));
+    VSG_HIP(vsg::launch_zero_u32(st, h->s_emit_over.p, 1));
 
     vsg::ShortScanArgs a;
     a.st = st;
@@ -424,6 +438,10 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     const uint32_t tstride = uint32_t(std::min<size_t>(16, std::max<size_t>(1, n_tiles / (4 * k))));
     const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
     a.tilemin = h->s_tilemin.p;
+    a.emit = h->s_emit.p;
+    a.wave_cap = wave_cap;
+    a.emit_count = h->s_emit_count.p;
+    a.overflow = h->s_emit_over.p;
     a.cand = h->s_cand.p;
     a.cap = cap;
     a.count = h->s_count.p;
@@ -447,6 +465,7 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
       VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
       ts.launches++;
     }
+    VSG_HIP(vsg::launch_scatter(p, a));
     {
       TimedScope ts(h, st, VSG_KERNEL_RESCORE);
       VSG_HIP(vsg::launch_rescore(st, h->metric == VSG_METRIC_L2, h->base, uint32_t(h->dp), uint32_t(h->dim),
@@ -455,7 +474,15 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
       ts.launches++;
     }
     VSG_HIP(hipMemcpyAsync(h->s_hover.p, h->s_nover.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VSG_HIP(hipMemcpyAsync(h->s_hover.p + 1, h->s_emit_over.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VSG_HIP(hipStreamSynchronize(st));
+    if (h->s_hover.p[1]) {  // an emission region overflowed: answer this round exactly
+      h->st_fallback += rq;
+      rc = enqueue_exact(h, qr, rq, k, VSG_METHOD_SKINNY, nullptr, ids, dists, st);
+      if (rc) return rc;
+      h->st_nq += rq;
+      continue;
+    }
     const uint32_t nover = h->s_hover.p[0];
     if (h->timing || nover) {
       counts.resize(rq);

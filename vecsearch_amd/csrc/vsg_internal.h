@@ -40,9 +40,13 @@ struct ShortScanArgs {
   uint32_t n_space;   // visited tiles: pass 1 ceil(n_tiles / tstride), pass 2 n_tiles
   uint32_t tstride;   // pass 1 visits every tstride-th tile; pass 2 uses 1
   uint32_t* tilemin;
-  uint64_t* cand;
+  uint64_t* emit;         // pass 2: per wave region (8 per tile), wave_cap entries
+  uint32_t wave_cap;
+  uint32_t* emit_count;   // per wave region
+  uint64_t* cand;         // per query, cap keys (filled by the scatter kernel)
   uint32_t cap;
   uint32_t* count;
+  uint32_t* overflow;     // blocks whose region overflowed
 };
 int short_ks(uint32_t d);
 int short_cand_cap();
@@ -60,6 +64,8 @@ hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq,
                               const unsigned long long* maxes, float* E);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a);
+hipError_t launch_scatter(int prec, const ShortScanArgs& a);
+int short_waves();  // waves per shortlist scan block (emission regions per tile)
 hipError_t launch_select_threshold(hipStream_t st, const uint32_t* tilemin, uint32_t n_tiles, uint32_t nq,
                                    const uint32_t* ks, uint32_t k, const float* E, float* thr);
 hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t dp, uint32_t d,

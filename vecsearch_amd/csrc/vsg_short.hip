@@ -195,7 +195,7 @@ __host__ __device__ constexpr int stage_group(int prec, int ks) {
 }
 
 // row fragments (16 rows each) per wave: the B operands of all k-steps stay in registers
-__host__ __device__ constexpr int rows_frags(int prec, int ks) { return ks * (prec == 1 ? 4 : 2) <= 48 ? 4 : 2; }
+__host__ __device__ constexpr int rows_frags(int prec, int ks) { return ks * (prec == 1 ? 4 : 3) <= 48 ? 4 : 2; }
 
 // Minimum over each group of 16 lanes (a DPP row), left in every lane of the row: two
 // quad_perm swaps, then rotations by 4 and 8 within the row.
@@ -249,12 +249,19 @@ struct StageShape {
   static_assert(PIECES % kWaves == 0, "stage pieces must split evenly over the waves");
 };
 
+// PASS 2 appends candidates to the wave's own region of `emit` (wave_cap entries, region
+// blockIdx.x * 8 + wave), each (ord(dh) << 32) | (query << 6) | row within the wave's 64 rows.
+// Slots come from a ballot and a wave-uniform count, and entries leave with plain stores:
+// no atomic sits in the MFMA loop (a returning atomic would make the wave wait for every
+// outstanding load, the LDS-DMA prefetch included). emit_count[region] receives the number of
+// entries wanted (more than wave_cap: the region overflowed). scatter_candidates_kernel then
+// files them per query.
 template <int PREC, int KS, int PASS>
 __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t tstride,
-    uint32_t n_space, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ cand, uint32_t cap,
-    uint32_t* __restrict__ count) {
+    uint32_t n_space, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ emit, uint32_t wave_cap,
+    uint32_t* __restrict__ emit_count) {
   typedef typename Frag<PREC>::T FT;
   typedef StageShape<PREC, KS> SH;
   constexpr int RF = SH::RF, G = SH::G, LOADS = SH::LOADS;
@@ -270,6 +277,8 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   const uint32_t tile = j * tstride;
   const uint32_t nrb = (n + 15) >> 4;
   const uint32_t rb0 = tile * (kWaves * RF) + wave * RF;  // first row block of this wave
+  uint64_t* wemit = emit + (size_t(j) * kWaves + wave) * wave_cap;
+  uint32_t n_emit = 0;  // wave-uniform
 
   // this wave's rows, all k-steps (zero for row blocks past the base)
   FT b[RF][KS];
@@ -280,10 +289,12 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     for (int s = 0; s < KS; ++s) b[f][s] = (rb0 + f < nrb) ? xf[(size_t(rb0 + f) * KS + s) * 64 + lane] : FT{};
   float xs[RF];
   bool ok[RF];
+  uint64_t okm[RF];  // wave masks of the rows inside the base
 #pragma unroll
   for (int f = 0; f < RF; ++f) {
     const uint32_t row = (rb0 + f) * 16 + (lane & 15);
     ok[f] = row < n;
+    okm[f] = ballot(ok[f]);
     xs[f] = (PREC == 2 && ok[f]) ? xscale[row] : 1.f;
   }
 
@@ -342,12 +353,11 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
         for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(a, b[f][s], acc[f]);
       }
       // value i of acc[f] belongs to query qf*16 + 4*(lane/16) + i and row (rb0+f)*16 + lane%16
+      const int q4 = 4 * (lane >> 4);
+      if (PASS == 1) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int qi = 4 * (lane >> 4) + i;  // query within the fragment
-        const uint32_t q = qf * 16 + qi;
-        const float qsc = PREC == 2 ? aux[g * 32 + 16 + qi] : 1.f;
-        if (PASS == 1) {
+        for (int i = 0; i < 4; ++i) {
+          const float qsc = PREC == 2 ? aux[g * 32 + 16 + q4 + i] : 1.f;
           float m = __builtin_inff();
 #pragma unroll
           for (int f = 0; f < RF; ++f) {
@@ -355,19 +365,43 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
             m = fminf(m, ok[f] ? 1.0f - sc : __builtin_inff());
           }
           m = row16_min(m);
-          if ((lane & 15) == 0) rd[wave * (G * 16) + g * 16 + qi] = m;
-        } else if (q < nq) {  // padding queries of the last stage never emit
-          const float t = aux[g * 32 + qi];
+          if ((lane & 15) == 0) rd[wave * (G * 16) + g * 16 + q4 + i] = m;
+        }
+      } else {
+        // aux holds score cuts: every row with 1 - score <= threshold has score >= cut (padding
+        // queries carry +inf and never emit). One mask test per fragment; the emission path
+        // runs only when some lane hits.
+        float cut[4], qsc[4], sc[RF][4];
 #pragma unroll
-          for (int f = 0; f < RF; ++f) {
-            const float sc = PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i];
-            const float dh = 1.0f - sc;
-            if (ok[f] && dh <= t) {
-              const uint32_t row = (rb0 + f) * 16 + (lane & 15);
-              const uint32_t slot_i = atomicAdd(count + q, 1u);
-              if (slot_i < cap) cand[size_t(q) * cap + slot_i] = make_key(dh, row);
-            }
+        for (int i = 0; i < 4; ++i) {
+          cut[i] = aux[g * 32 + q4 + i];
+          qsc[i] = PREC == 2 ? aux[g * 32 + 16 + q4 + i] : 1.f;
+        }
+        uint64_t any = 0;
+#pragma unroll
+        for (int f = 0; f < RF; ++f)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            sc[f][i] = PREC == 2 ? acc[f][i] * qsc[i] * xs[f] : acc[f][i];
+            any |= ballot(sc[f][i] >= cut[i]) & okm[f];
           }
+        if (any) {
+#pragma unroll
+          for (int f = 0; f < RF; ++f)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const bool hit = ok[f] && sc[f][i] >= cut[i];
+              const uint64_t m = ballot(hit);
+              if (m) {
+                const uint32_t pre =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+                const uint32_t at = n_emit + pre;
+                const uint32_t q = qf * 16 + q4 + i;
+                if (hit && at < wave_cap)
+                  wemit[at] = make_key(1.0f - sc[f][i], (q << 6) | uint32_t(f * 16 + (lane & 15)));
+                n_emit += uint32_t(__builtin_popcountll(m));
+              }
+            }
         }
       }
     }
@@ -386,16 +420,62 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     __syncthreads();
     flush_min(nstages - 1);
   }
+  if (PASS == 2 && lane == 0) emit_count[size_t(j) * kWaves + wave] = n_emit;
+}
+
+// Files the pass-2 entries of every wave region into per-query candidate lists (count[q] keys
+// (ord(dh) << 32) | row at cand[q * cap ..]); a region that overflowed sets *overflow (the round
+// is then answered by the exact scan). One block of 256 threads per tile of 8 wave regions;
+// region r covers rows r * rows_per_wave ...
+__global__ __launch_bounds__(256) void scatter_candidates_kernel(
+    const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
+    uint32_t rows_per_wave, uint64_t* __restrict__ cand, uint32_t cap, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ overflow) {
+  for (int w = 0; w < kWaves; ++w) {
+    const size_t r = size_t(blockIdx.x) * kWaves + w;
+    const uint32_t c = emit_count[r];
+    if (c > wave_cap) {
+      if (threadIdx.x == 0) atomicAdd(overflow, 1u);
+      continue;
+    }
+    for (uint32_t i = threadIdx.x; i < c; i += 256) {
+      const uint64_t e = emit[r * wave_cap + i];
+      const uint32_t lo = uint32_t(e), q = lo >> 6;
+      const uint32_t row = uint32_t(r) * rows_per_wave + (lo & 63);
+      const uint32_t slot = atomicAdd(count + q, 1u);
+      if (slot < cap) cand[size_t(q) * cap + slot] = (e & 0xFFFFFFFF00000000ull) | row;
+    }
+  }
+}
+
+// Smallest float above or equal to the real number s (s finite or infinite, not NaN).
+__device__ __forceinline__ float round_up_f(double s) {
+  float r = float(s);
+  if (double(r) < s) r = __uint_as_float(r >= 0.f ? __float_as_uint(r) + 1u : __float_as_uint(r) - 1u);
+  return r;
+}
+__device__ __forceinline__ float round_down_f(double s) { return -round_up_f(-s); }
+
+// Score cut of a distance threshold T: every float score s with fl(1 - s) <= T has s >= cut.
+// fl(1 - s) = (1 - s)(1 + d), |d| <= 2^-24, so fl(1 - s) <= T gives 1 - s <= T + |T| 2^-23
+// and s >= 1 - T - |T| 2^-23; the cut takes twice that margin and rounds down. T = -inf (no
+// candidates) gives +inf; T = +inf or NaN gives -inf (every row is a candidate).
+__device__ __forceinline__ float score_cut(float T) {
+  if (T == -__builtin_inff()) return __builtin_inff();
+  if (!(T < __builtin_inff())) return -__builtin_inff();
+  const double t = T;
+  return round_down_f(1.0 - t - fabs(t) * 0x1p-22 - 0x1p-60);
 }
 
 // Builds the per-fragment auxiliary block of the query stages: for every fragment f,
-// aux[f * 32 + i] = thr[q] and aux[f * 32 + 16 + i] = qscale[q] (q = 16 f + i; padding -inf / 1).
+// aux[f * 32 + i] = score_cut(thr[q]) and aux[f * 32 + 16 + i] = qscale[q] (q = 16 f + i;
+// padding queries +inf / 1).
 __global__ void query_aux_kernel(const float* __restrict__ thr, const float* __restrict__ qscale,
                                  uint32_t nq, uint32_t nfrag_pad, float* __restrict__ aux) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nfrag_pad * 16) return;
   const uint32_t f = t >> 4, i = t & 15, q = t;
-  aux[f * 32 + i] = (thr && q < nq) ? thr[q] : -__builtin_inff();
+  aux[f * 32 + i] = (thr && q < nq) ? score_cut(thr[q]) : __builtin_inff();
   aux[f * 32 + 16 + i] = (qscale && q < nq) ? qscale[q] : 1.f;
 }
 
@@ -440,8 +520,8 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
   }
   if (tid == 0) {
     const float U = unord_bits(prefix);
-    // absolute cushion 2^-20 covers the rounding of this sum
-    thr[q] = U + 2.0f * E[q] * (1.0f + 1e-6f) + 9.5367431640625e-07f;
+    // U + 2E (with a relative cushion on E and an absolute one of 2^-20), rounded up
+    thr[q] = round_up_f(double(U) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
   }
 }
 
@@ -502,7 +582,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   if (tid == 0) nsel = 0;
   __syncthreads();
   if (kk > 0) {
-    const float lim = key_dist(cbuf[kk - 1]) + 2.0f * E[q] * (1.0f + 1e-6f) + 9.5367431640625e-07f;
+    const float lim = round_up_f(double(key_dist(cbuf[kk - 1])) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
     for (uint32_t i = tid; i < c; i += 256)
       if (key_dist(cbuf[i]) <= lim) atomicMax(&nsel, i + 1);
   }
@@ -581,6 +661,14 @@ hipError_t launch_query_aux(hipStream_t st, const float* thr, const float* qscal
   return hipGetLastError();
 }
 
+int short_waves() { return kWaves; }
+
+hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
+  hipLaunchKernelGGL(scatter_candidates_kernel, dim3(a.n_blocks), dim3(256), 0, a.st, a.emit, a.emit_count,
+                     a.wave_cap, uint32_t(rows_frags(prec, a.KS) * 16), a.cand, a.cap, a.count, a.overflow);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(unsigned(std::min<size_t>(1024, (n + 255) / 256))), dim3(256), 0,
@@ -595,10 +683,12 @@ hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
   const dim3 grid(a.n_blocks), block(64 * kWaves);
   if (pass == 1)
     hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.cand, a.cap, a.count);
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.emit, a.wave_cap,
+                       a.emit_count);
   else
     hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.cand, a.cap, a.count);
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.emit, a.wave_cap,
+                       a.emit_count);
   return hipGetLastError();
 }
 
