@@ -9,12 +9,17 @@ the timed region; the PCIe-inclusive rate (host queries in, host results out, as
 reference's bench/gpu_bench.py:75-85 times it) is reported per batch size in "sweep".
 
 One JSON line on stdout carries the contract fields plus:
-  roofline      the dominant kernel (the exact streaming scan) against HBM bandwidth: algorithmic
-                bytes per launch = N*d*4 + b*d*4 + b*k*8 (b = queries per pass) over its average
-                launch time, measured with HIP events on the launch stream during the timed steps;
+  roofline      the dominant kernel of the timed steps (at batch 1 the exact streaming scan)
+                against HBM bandwidth: algorithmic bytes per launch = N*d*4 + b*d*4 + b*k*8
+                (b = queries per pass) over its average launch time, measured with HIP events
+                on the launch stream during the timed steps;
   cpu_baseline  the reference CPU index (oracle/_ref, BruteForceIndex::search_batch) or the C
                 oracle timed on this host's cores over a bounded sample of the same workload;
-  sweep         per batch size: device-resident QPS, and PCIe-inclusive QPS / p50 / p99 ms.
+  shortlist     the BF16 and FP8 certified-shortlist forms at a large batch: QPS, the MFMA scan
+                kernel against the dense MFMA peak (algorithmic flops 2*b*N*d per launch), the
+                id identity with the plain exact path, and the FP64 score error;
+  sweep         per batch size: device-resident QPS of the plain exact path and of the two
+                shortlist forms, and the PCIe-inclusive QPS / p50 / p99 ms of method "auto".
 """
 import argparse
 import json
@@ -29,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector packed / matrix)
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0}  # MI355X dense matrix peaks
 
 
 def log(*a):
@@ -48,7 +54,13 @@ def parse():
     ap.add_argument("--method", default="auto")
     ap.add_argument("--sweep", default="1,8,32,128,512,1024,4096",
                     help="batch sizes for the per-batch table ('' to skip)")
+    ap.add_argument("--short-batch", type=int, default=4096,
+                    help="batch of the shortlist measurement (0 to skip)")
+    ap.add_argument("--short-seconds", type=float, default=2.0, help="timing budget per shortlist form")
     ap.add_argument("--sweep-seconds", type=float, default=0.5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run 5 device-to-device copies of MB megabytes (the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings), then exit")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
     return ap.parse_args()
@@ -85,7 +97,7 @@ def cpu_baseline(base, queries, k, seconds, threads):
         run()
         calls += 1
         el = time.perf_counter() - t0
-        if el >= seconds or calls >= 200:
+        if el >= seconds or calls >= 5000:
             break
     qps = calls * nq / el
     return {"value": round(qps, 2), "unit": "queries/s", "cores": threads, "kind": kind,
@@ -102,6 +114,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 copies of {src.numel() * 4:,} bytes")
+        return
 
     import vecsearch_amd as vs
     from vecsearch_amd import datagen
@@ -122,7 +144,7 @@ def main():
 
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(queries).to(dev)
-    maxb = max([B] + [int(b) for b in args.sweep.split(",") if b])
+    maxb = max([B, args.short_batch] + [int(b) for b in args.sweep.split(",") if b])
     ids = torch.empty((maxb, k), dtype=torch.int32, device=dev)
     dists = torch.empty((maxb, k), dtype=torch.float32, device=dev)
     # a dedicated (default-priority) stream: kernels and timing events share it
@@ -169,43 +191,109 @@ def main():
         achieved = alg_flops / (scan_ms / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4)}
-    roof["traffic"] = traffic_from_profiles(f"scan_exact_{n // 1000}kx{d}_b{qb}_k{k}_traffic.md")
+    roof["traffic"] = traffic_from_profiles(f"scan_exact_{n // 1000}kx{d}_b{qb}_k{k}.md")
     roof["kernel"] = "vsg::scan_exact_kernel"
     roof["kernel_ms"] = round(scan_ms, 4)
     roof["alg_bytes_per_launch"] = alg_bytes
     roof["merge_ms"] = round(kt["merge_ms"] / max(1, kt["merge_launches"]), 4)
+
+    def device_qps(b, method, seconds):
+        """QPS of method at batch b, queries and results in HBM; kernel times of the same calls."""
+        step_m(0, b, method)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_m(1, b, method)
+        torch.cuda.synchronize()
+        one = time.perf_counter() - t0
+        reps = int(min(200, max(2, seconds / max(one, 1e-6))))
+        idx.kernel_timing(True)
+        idx.kernel_times()
+        e0.record(stream)
+        for r in range(reps):
+            step_m(r, b, method)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        kt = idx.kernel_times()
+        idx.kernel_timing(False)
+        return b * reps / (e0.elapsed_time(e1) / 1e3), kt
+
+    def step_m(i, b, method):
+        off = (i * b) % (nqt - b + 1)
+        idx.search_device(dq.data_ptr() + off * d * 4, b, k, ids.data_ptr(), dists.data_ptr(), sptr, method)
+
+    def fp64_error(qs, got_ids, got_d):
+        """Largest |distance - FP64 distance| over the returned ids (cosine: 1 - <q, x>)."""
+        err = 0.0
+        for i in range(len(qs)):
+            sel = got_ids[i][got_ids[i] >= 0]
+            ex = 1.0 - base[sel].astype(np.float64) @ qs[i].astype(np.float64)
+            err = max(err, float(np.max(np.abs(ex - got_d[i][:len(sel)].astype(np.float64)))) if len(sel) else 0.0)
+        return err
+
+    # ---- shortlist forms at a large batch ----
+    shortlist = {}
+    sb = min(args.short_batch, nqt)
+    if sb > 0:
+        nchk = min(sb, 256)
+        step_m(0, nchk, "skinny")
+        torch.cuda.synchronize()
+        plain_ids = ids[:nchk].cpu().numpy().copy()
+        plain_d = dists[:nchk].cpu().numpy().copy()
+        for m in ("bf16", "fp8"):
+            qps, kt = device_qps(sb, m, args.short_seconds)
+            step_m(0, sb, m)
+            torch.cuda.synchronize()
+            got_ids = ids[:nchk].cpu().numpy()
+            got_d = dists[:nchk].cpu().numpy()
+            scan_ms = kt["mfma_ms"] / max(1, kt["mfma_launches"])
+            flops = 2.0 * sb * n * d
+            ach = flops / (scan_ms / 1e3) / 1e12
+            idx.kernel_timing(True)  # candidate statistics are gathered with timing on
+            step_m(0, sb, m)
+            torch.cuda.synchronize()
+            st = idx.short_stats()
+            idx.kernel_times()
+            idx.kernel_timing(False)
+            shortlist[m] = {
+                "batch": sb, "qps": round(qps, 1), "ms_per_batch": round(sb / qps * 1e3, 4),
+                "roofline": {"bound": "mfma", "kernel": f"vsg::mfma_scan_kernel<{1 if m == 'bf16' else 2},KS,2>",
+                             "achieved": round(ach, 1), "peak": MFMA_PEAK_TFLOPS[m], "unit": "TFLOP/s",
+                             "frac": round(ach / MFMA_PEAK_TFLOPS[m], 4), "kernel_ms": round(scan_ms, 4)},
+                "kernel_ms_per_batch": {kk[:-3]: round(v / max(1, kt["mfma_launches"]), 4)
+                                        for kk, v in kt.items() if kk.endswith("_ms") and v > 0},
+                "candidates_per_query": round(st["emitted"] / max(1, sb), 1),
+                "rescored_per_query": round(st["rescored"] / max(1, sb), 1),
+                "exact_fallbacks": st["fallbacks"],
+                "ids_identical_to_plain": bool(np.array_equal(got_ids, plain_ids)),
+                "dists_bit_identical_to_plain": bool(np.array_equal(got_d.view(np.uint32), plain_d.view(np.uint32))),
+                "checked_queries": nchk,
+            }
+            log(f"shortlist {m}: {shortlist[m]}")
+        shortlist["fp64_max_abs_err"] = fp64_error(queries[:nchk], plain_ids, plain_d)
 
     # ---- sweep: device-resident QPS and host (PCIe-inclusive) latency per batch ----
     sweep = {}
     for b in [int(x) for x in args.sweep.split(",") if x]:
         if b > nqt:
             continue
-        step(0, b)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        step(1, b)
-        torch.cuda.synchronize()
-        one = time.perf_counter() - t0
-        reps = int(min(200, max(3, args.sweep_seconds / 2 / max(one, 1e-6))))
-        e0.record(stream)
-        for r in range(reps):
-            step(r, b)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        dev_qps = b * reps / (e0.elapsed_time(e1) / 1e3)
+        row = {}
+        row["qps_plain"] = round(device_qps(b, "skinny", args.sweep_seconds)[0], 1)
+        if b >= 32:
+            for m in ("bf16", "fp8"):
+                row[f"qps_{m}"] = round(device_qps(b, m, args.sweep_seconds)[0], 1)
         lat = []
         qh = queries
-        t0 = time.perf_counter()
         while len(lat) < 3 or sum(lat) < args.sweep_seconds:
             off = (len(lat) * b) % (nqt - b + 1)
             t1 = time.perf_counter()
             idx.search(qh[off:off + b], k)
             lat.append(time.perf_counter() - t1)
         lat = np.array(lat)
-        sweep[str(b)] = {"qps_device": round(dev_qps, 1), "qps_host": round(b / float(np.median(lat)), 1),
-                         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "calls": len(lat)}
-        log(f"  sweep batch {b}: {sweep[str(b)]}")
+        row.update({"qps_host_auto": round(b / float(np.median(lat)), 1),
+                    "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "calls": len(lat)})
+        sweep[str(b)] = row
+        log(f"  sweep batch {b}: {row}")
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -232,6 +320,7 @@ def main():
                    "queries": nqt, "method": args.method, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "shortlist": shortlist,
         "sweep": sweep,
     }
     print(json.dumps(out), flush=True)

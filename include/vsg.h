@@ -116,9 +116,10 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value);
  * (arrays of VSG_KERNEL_KINDS entries), and resets them. */
 #define VSG_KERNEL_SCAN 0    /* exact streaming scan passes */
 #define VSG_KERNEL_MERGE 1   /* per-query merge of the scan's lists */
-#define VSG_KERNEL_MFMA 2    /* shortlist MFMA passes (tile minima, emit) */
+#define VSG_KERNEL_MFMA 2    /* shortlist MFMA scan over every row (candidate emission) */
 #define VSG_KERNEL_RESCORE 3 /* shortlist exact rescoring */
-#define VSG_KERNEL_KINDS 4
+#define VSG_KERNEL_SAMPLE 4  /* shortlist sampled MFMA pass, threshold select, candidate filing */
+#define VSG_KERNEL_KINDS 5
 int vsg_kernel_timing(vsg_index* h, int enable);
 int vsg_kernel_times(vsg_index* h, double* ms, uint64_t* launches);
 

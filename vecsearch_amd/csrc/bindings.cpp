@@ -180,7 +180,7 @@ PYBIND11_MODULE(_vecsearch, m) {
              uint64_t n[VSG_KERNEL_KINDS];
              if (vsg_kernel_times(self.handle(), ms, n) != VSG_OK) throw std::runtime_error(vsg_last_error());
              py::dict d;
-             const char* names[VSG_KERNEL_KINDS] = {"scan", "merge", "mfma", "rescore"};
+             const char* names[VSG_KERNEL_KINDS] = {"scan", "merge", "mfma", "rescore", "sample"};
              for (int i = 0; i < VSG_KERNEL_KINDS; ++i) {
                d[(std::string(names[i]) + "_ms").c_str()] = ms[i];
                d[(std::string(names[i]) + "_launches").c_str()] = n[i];

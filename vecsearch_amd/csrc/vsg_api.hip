@@ -448,14 +448,14 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     VSG_HIP(vsg::launch_query_aux(st, nullptr, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
                                   h->s_qaux.p));
     {
-      TimedScope ts(h, st, VSG_KERNEL_MFMA);
+      TimedScope ts(h, st, VSG_KERNEL_SAMPLE);
       a.tstride = tstride;
       a.n_space = a.n_blocks = space1;
       VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
+      VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, space1, rq, nullptr, uint32_t(k), h->s_E.p,
+                                           h->s_thr.p));
       ts.launches++;
     }
-    VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, space1, rq, nullptr, uint32_t(k), h->s_E.p,
-                                         h->s_thr.p));
     VSG_HIP(vsg::launch_query_aux(st, h->s_thr.p, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
                                   h->s_qaux.p));
     {
@@ -465,7 +465,10 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
       VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
       ts.launches++;
     }
-    VSG_HIP(vsg::launch_scatter(p, a));
+    {
+      TimedScope ts(h, st, VSG_KERNEL_SAMPLE);
+      VSG_HIP(vsg::launch_scatter(p, a));
+    }
     {
       TimedScope ts(h, st, VSG_KERNEL_RESCORE);
       VSG_HIP(vsg::launch_rescore(st, h->metric == VSG_METRIC_L2, h->base, uint32_t(h->dp), uint32_t(h->dim),
@@ -759,8 +762,8 @@ int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   std::lock_guard<std::mutex> g(h->mu);
   VSG_HIP(hipSetDevice(h->device));
-  double ms[VSG_KERNEL_KINDS] = {0, 0, 0, 0};
-  uint64_t cnt[VSG_KERNEL_KINDS] = {0, 0, 0, 0};
+  double ms[VSG_KERNEL_KINDS] = {};
+  uint64_t cnt[VSG_KERNEL_KINDS] = {};
   hipError_t err = hipSuccess;
   for (auto& t : h->timed) {
     float v = 0;
