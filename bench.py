@@ -58,6 +58,9 @@ def parse():
                     help="batch of the shortlist measurement (0 to skip)")
     ap.add_argument("--short-seconds", type=float, default=2.0, help="timing budget per shortlist form")
     ap.add_argument("--sweep-seconds", type=float, default=0.5)
+    ap.add_argument("--filtered", type=int, default=4096,
+                    help="queries of the batched filtered scan measurement (0 to skip)")
+    ap.add_argument("--filtered-rows", type=int, default=892_800)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB megabytes (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings), then exit")
@@ -271,6 +274,61 @@ def main():
             log(f"shortlist {m}: {shortlist[m]}")
         shortlist["fp64_max_abs_err"] = fp64_error(queries[:nchk], plain_ids, plain_d)
 
+    # ---- batched filtered exact scan: Zipf groups, one persistent launch ----
+    filtered = {}
+    nf = min(args.filtered_rows, n)
+    if args.filtered > 0 and nf > 0:
+        groups = datagen.zipf_groups(nf, 100, 1.1, seed=2)
+        goff, gids = datagen.group_csr(groups, 100)
+        fq = datagen.mixture(args.filtered, d, seed=4)
+        fks, fg = datagen.query_filters(args.filtered, 100, seed=3, ks=(10, 100))
+        franges = datagen.query_ranges(fg, goff)
+        sizes = np.diff(goff)
+        t0 = time.perf_counter()
+        fres = idx.search_batch(fq, fks, franges, gids)  # first call: the autotuner picks the tile
+        tune_s = time.perf_counter() - t0
+        lat = []
+        while len(lat) < 5 or sum(lat) < args.sweep_seconds:
+            t1 = time.perf_counter()
+            idx.search_batch(fq, fks, franges, gids)
+            lat.append(time.perf_counter() - t1)
+        idx.kernel_timing(True)
+        idx.kernel_times()
+        reps = 3
+        for _ in range(reps):
+            idx.search_batch(fq, fks, franges, gids)
+        kt = idx.kernel_times()
+        idx.kernel_timing(False)
+        lists_ms = kt["lists_ms"] / reps
+        merge_ms = kt["merge_ms"] / reps
+        pairs = int(np.sum(franges[:, 1] - franges[:, 0]))
+        used = np.unique(fg)
+        alg_bytes = int(np.sum(sizes[used])) * d * 4 + args.filtered * d * 4 + int(np.sum(fks)) * 8
+        flops = 2.0 * d * pairs
+        lat = np.array(lat)
+        filtered = {
+            "workload": f"{nf:,} x {d} rows (the first rows of the base), 100 Zipf(s=1.1) groups; "
+                        f"{args.filtered} queries, each one uniform group, k uniform in {{10, 100}}",
+            "group_fraction_min": round(float(sizes.min()) / nf, 6),
+            "group_fraction_max": round(float(sizes.max()) / nf, 6),
+            "qps_host": round(args.filtered / float(np.median(lat)), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "first_call_s": round(tune_s, 3),
+            "tile": {"qb": idx.get_param("lists_last_qb"), "r": idx.get_param("lists_last_r"),
+                     "chunk": idx.get_param("lists_last_chunk")},
+            "lists_kernel_ms": round(lists_ms, 4),
+            "merge_kernel_ms": round(merge_ms, 4),
+            "query_row_pairs": pairs,
+            "roofline": {"bound": "valu", "achieved": round(flops / (lists_ms / 1e3) / 1e12, 2),
+                         "peak": 78.6, "unit": "TFLOP/s (FP32 FMA, non-packed)",
+                         "frac": round(flops / (lists_ms / 1e3) / 1e12 / 78.6, 4),
+                         "hbm_alg_bytes": alg_bytes,
+                         "hbm_frac_of_alg_bytes": round(alg_bytes / (lists_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        }
+        del fres
+        log(f"filtered: {filtered}")
+
     # ---- sweep: device-resident QPS and host (PCIe-inclusive) latency per batch ----
     sweep = {}
     for b in [int(x) for x in args.sweep.split(",") if x]:
@@ -321,6 +379,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "shortlist": shortlist,
+        "filtered": filtered,
         "sweep": sweep,
     }
     print(json.dumps(out), flush=True)

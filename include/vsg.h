@@ -100,15 +100,29 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
  * returns ks[i] <= kmax results. Ranges may coincide: queries with identical ranges share
  * their row reads. n_ids = len(allow_ids) (every range must lie inside it). Outputs are
  * nq x kmax with columns ks[i].. padded. Host buffers. This is HNSWIndex::search's
- * exact_below path (hnsw.cpp:322-354) for many queries with different filters in one call. */
+ * exact_below path (hnsw.cpp:322-354) for many queries with different filters in one call.
+ * A query whose range begins with VSG_ALL_ROWS (its end is ignored) is unfiltered: it scans
+ * every row, in the same persistent launch as the filtered queries (GpuBruteForceIndex::search
+ * semantics, gpu_brute_force.cu:425-488). */
+#define VSG_ALL_ROWS 0xFFFFFFFFFFFFFFFFull
 int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
                      size_t kmax, const uint64_t* range, const uint32_t* allow_ids, size_t n_ids,
                      uint32_t* out_ids, float* out_dists);
 
-/* Launch-shape parameters (the tile table an autotuner fills). Known names:
- *   "scan_waves_per_cu"  waves per CU of the streaming scan (1..32, default 8);
- *   "auto_short_batch"   smallest batch that AUTO sends to the shortlist (default 32). */
+/* Launch-shape parameters. Known names:
+ *   "scan_waves_per_cu"    waves per CU of the streaming scan (1..32, default 8);
+ *   "auto_short_batch"     smallest batch that AUTO sends to the shortlist (default 32);
+ *   "lists_qb", "lists_r", "lists_chunk", "lists_blocks_per_cu"
+ *                          a fixed tile of the filtered scan (queries per work item 1/2/4/8/16,
+ *                          16-row groups per lane step 2/4, allowed rows per item, blocks per
+ *                          CU); 0 = chosen per call (default);
+ *   "lists_autotune"       1 (default): with no fixed tile, the first large call of each shape
+ *                          class times the candidate tiles on its own inputs and the class keeps
+ *                          the fastest; 0: the planner's default tile, and the table is cleared.
+ * vsg_get_param reads these, and "lists_tuned_classes" (size of the tuned table) and
+ * "lists_last_qb" / "lists_last_r" / "lists_last_chunk" (the tile of the last filtered call). */
 int vsg_set_param(vsg_index* h, const char* name, int64_t value);
+int vsg_get_param(vsg_index* h, const char* name, int64_t* value);
 
 /* Kernel timing for measurement: while enabled, every search records HIP events around its
  * kernel launches on the stream they run on, by kind. vsg_kernel_times waits for the recorded
@@ -119,7 +133,8 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value);
 #define VSG_KERNEL_MFMA 2    /* shortlist MFMA scan over every row (candidate emission) */
 #define VSG_KERNEL_RESCORE 3 /* shortlist exact rescoring */
 #define VSG_KERNEL_SAMPLE 4  /* shortlist sampled MFMA pass, threshold select, candidate filing */
-#define VSG_KERNEL_KINDS 5
+#define VSG_KERNEL_LISTS 5   /* filtered scan over allowed-id lists (persistent launch) */
+#define VSG_KERNEL_KINDS 6
 int vsg_kernel_timing(vsg_index* h, int enable);
 int vsg_kernel_times(vsg_index* h, double* ms, uint64_t* launches);
 

@@ -130,8 +130,12 @@ def search(x, q, k, metric="l2", threads=1, mask=None):
     return ids, dists
 
 
+ALL_ROWS = 2**64 - 1  # a range beginning with this value is unfiltered (every row)
+
+
 def search_lists(x, q, ks, ranges, allow_ids, metric="l2", threads=1, kmax=None):
-    """Per-query k and allowed-id lists (query i scans allow_ids[ranges[i,0]:ranges[i,1]])."""
+    """Per-query k and allowed-id lists (query i scans allow_ids[ranges[i,0]:ranges[i,1]], or
+    every row when ranges[i,0] == ALL_ROWS)."""
     x, q = _f32(x), _f32(q)
     ks = np.ascontiguousarray(ks, dtype=np.uint32)
     off = np.ascontiguousarray(ranges, dtype=np.uint64).reshape(-1)

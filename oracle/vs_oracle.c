@@ -140,6 +140,7 @@ typedef struct {
   const uint32_t* ks;      /* optional per-query k (lists mode) */
   const uint64_t* off;     /* lists mode: [begin, end) per query */
   const uint32_t* ids;     /* lists mode */
+  size_t n;                /* lists mode: rows of an unfiltered query */
   size_t lo, hi;           /* row range (batch / masked mode) or query range (lists mode) */
   nb_t* heaps;             /* nq x k for this thread (batch / masked mode) */
   size_t* sizes;           /* nq (batch / masked mode) */
@@ -226,8 +227,11 @@ static void* scan_lists(void* arg) {
     const size_t kq = j->ks[q] < j->kstride ? j->ks[q] : j->kstride;
     size_t m = 0;
     const float* qv = j->queries + q * j->d;
-    for (uint64_t p = j->off[2 * q]; p < j->off[2 * q + 1]; ++p) {
-      const uint32_t id = j->ids[p];
+    /* a range beginning with UINT64_MAX is unfiltered: every row, in id order */
+    const int all = j->off[2 * q] == UINT64_MAX;
+    const uint64_t pb = all ? 0 : j->off[2 * q], pe = all ? j->n : j->off[2 * q + 1];
+    for (uint64_t p = pb; p < pe; ++p) {
+      const uint32_t id = all ? (uint32_t)p : j->ids[p];
       nb_t c = {vso_distance(j->metric, qv, j->data + (size_t)id * j->d, j->d), id};
       heap_push(h, &m, kq, c);
     }
@@ -246,7 +250,6 @@ void vso_search_lists(const float* data, size_t n, size_t d, int metric, const f
                       size_t nq, const uint32_t* ks, size_t kmax, const uint64_t* off,
                       const uint32_t* ids, uint32_t* out_ids, float* out_dists, int nthreads) {
   /* off: 2 * nq range bounds */
-  (void)n;
   if (nq == 0 || kmax == 0) return;
   size_t nt = nthreads < 1 ? 1 : (size_t)nthreads;
   if (nt > nq) nt = nq;
@@ -254,7 +257,7 @@ void vso_search_lists(const float* data, size_t n, size_t d, int metric, const f
   for (size_t t = 0; t < nt; ++t) {
     job_t* j = &jobs[t];
     j->data = data; j->d = d; j->metric = metric; j->queries = queries; j->nq = nq;
-    j->k = kmax; j->kstride = kmax; j->ks = ks; j->off = off; j->ids = ids;
+    j->k = kmax; j->kstride = kmax; j->ks = ks; j->off = off; j->ids = ids; j->n = n;
     j->lo = nq * t / nt; j->hi = nq * (t + 1) / nt;
     j->out_ids = out_ids; j->out_dists = out_dists;
   }

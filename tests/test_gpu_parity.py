@@ -159,6 +159,54 @@ def test_search_batch_lists(metric):
     assert_same(got, exp)
 
 
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_search_batch_mixed_unfiltered(metric):
+    # unfiltered queries (ALL_ROWS) run in the same persistent launch as filtered ones
+    n, d = 30000, 64
+    x = dg.mixture(n, d, seed=24)
+    groups = dg.zipf_groups(n, 20, 1.1, seed=25)
+    off, ids_g = dg.group_csr(groups, 20)
+    q = dg.mixture(90, d, seed=26)
+    ks, qg = dg.query_filters(90, 20, seed=27, ks=(1, 10, 100))
+    ranges = dg.query_ranges(qg, off)
+    ranges[::3] = [vs.ALL_ROWS, 0]
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    got = gpu.search_batch(q, ks, ranges, ids_g)
+    assert_same(got, oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=100))
+    # and an unfiltered query equals the plain search
+    plain = gpu.search(q[0:1], int(ks[0]), method="skinny")
+    np.testing.assert_array_equal(got[0][0, :ks[0]], plain[0][0])
+
+
+def test_search_batch_autotuner_keeps_results():
+    # the first large call of a shape class times every candidate tile on its own inputs; the
+    # class then reuses the fastest; results are the same bits whichever tile runs
+    n, d = 60000, 64
+    x = dg.mixture(n, d, seed=28)
+    q = dg.mixture(300, d, seed=29)
+    ks = np.full(300, 10, np.uint32)
+    ranges = np.zeros((300, 2), np.uint64)
+    ranges[:, 0] = vs.ALL_ROWS
+    allow = np.zeros(1, np.uint32)
+    gpu = vs.GpuBruteForceIndex(d, "ip")
+    gpu.add(x)
+    assert gpu.get_param("lists_tuned_classes") == 0
+    first = gpu.search_batch(q, ks, ranges, allow)
+    assert gpu.get_param("lists_tuned_classes") == 1
+    assert_same(first, oracle.search_lists(x, q, ks, ranges, allow, "ip", threads=8, kmax=10))
+    second = gpu.search_batch(q, ks, ranges, allow)
+    assert_same(second, (np.where(first[0] < 0, oracle.NONE, first[0]).astype(np.uint32), first[1]))
+    assert gpu.get_param("lists_last_qb") in (1, 2, 4, 8, 16)
+    gpu.set_param("lists_autotune", 0)
+    assert gpu.get_param("lists_tuned_classes") == 0
+    gpu.set_param("lists_qb", 4)
+    gpu.set_param("lists_r", 4)
+    third = gpu.search_batch(q, ks, ranges, allow)
+    assert gpu.get_param("lists_last_qb") == 4 and gpu.get_param("lists_last_r") == 4
+    np.testing.assert_array_equal(third[0], first[0])
+
+
 def test_search_batch_edge_cases():
     x = data(3000, 20, 30)
     gpu = vs.GpuBruteForceIndex(20)

@@ -174,6 +174,23 @@ def test_lists_equal_masked_search():
         assert (ids[j, ks[j]:] == oracle.NONE).all() and np.isinf(dists[j, ks[j]:]).all()
 
 
+def test_lists_unfiltered_rows_equal_plain_search():
+    # a range beginning with ALL_ROWS scans every row: the same answer as the unfiltered search
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((1500, 20)).astype(np.float32)
+    q = rng.standard_normal((6, 20)).astype(np.float32)
+    allow = np.array([2, 40, 41, 900], np.uint32)
+    ranges = np.array([[oracle.ALL_ROWS, 0], [0, 4], [oracle.ALL_ROWS, 7], [1, 3], [oracle.ALL_ROWS, 0],
+                       [0, 0]], np.uint64)
+    ks = np.array([10, 3, 1, 2, 0, 5], np.uint32)
+    ids, dists = oracle.search_lists(x, q, ks, ranges, allow, "l2", threads=2, kmax=10)
+    for j in (0, 2):
+        ei, ed = oracle.search(x, q[j], int(ks[j]), "l2")
+        assert np.array_equal(ids[j, :ks[j]], ei[0]) and np.array_equal(bits(dists[j, :ks[j]]), bits(ed[0]))
+    assert (ids[4] == oracle.NONE).all() and (ids[5] == oracle.NONE).all()
+    assert set(ids[1, :3]) <= {2, 40, 41, 900}
+
+
 # ---- 4. planted mistakes fail the golden vectors ------------------------------------------
 
 def _model_dot(a, b, mistake=None):

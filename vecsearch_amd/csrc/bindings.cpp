@@ -144,7 +144,8 @@ PYBIND11_MODULE(_vecsearch, m) {
           py::arg("queries"), py::arg("ks"), py::arg("ranges"), py::arg("allow_ids"),
           py::arg("kmax") = py::none(),
           "Batched filtered exact scan. Query i returns ks[i] results among ids\n"
-          "allow_ids[ranges[i,0]:ranges[i,1]] (strictly increasing); queries with identical\n"
+          "allow_ids[ranges[i,0]:ranges[i,1]] (strictly increasing), or every row when\n"
+          "ranges[i,0] == ALL_ROWS (unfiltered, same launch); queries with identical\n"
           "ranges share row reads. Returns (ids[int64, nq x kmax], distances[float32, nq x kmax]);\n"
           "columns past ks[i] are -1 / inf.")
       .def(
@@ -180,7 +181,7 @@ PYBIND11_MODULE(_vecsearch, m) {
              uint64_t n[VSG_KERNEL_KINDS];
              if (vsg_kernel_times(self.handle(), ms, n) != VSG_OK) throw std::runtime_error(vsg_last_error());
              py::dict d;
-             const char* names[VSG_KERNEL_KINDS] = {"scan", "merge", "mfma", "rescore", "sample"};
+             const char* names[VSG_KERNEL_KINDS] = {"scan", "merge", "mfma", "rescore", "sample", "lists"};
              for (int i = 0; i < VSG_KERNEL_KINDS; ++i) {
                d[(std::string(names[i]) + "_ms").c_str()] = ms[i];
                d[(std::string(names[i]) + "_launches").c_str()] = n[i];
@@ -207,6 +208,15 @@ PYBIND11_MODULE(_vecsearch, m) {
               throw std::invalid_argument(vsg_last_error());
           },
           py::arg("name"), py::arg("value"), "Set a launch-shape parameter (see vsg.h).")
+      .def(
+          "get_param",
+          [](GpuBruteForceIndex& self, const std::string& name) {
+            int64_t v = 0;
+            if (vsg_get_param(self.handle(), name.c_str(), &v) != VSG_OK)
+              throw std::invalid_argument(vsg_last_error());
+            return v;
+          },
+          py::arg("name"), "Read a launch-shape parameter or tuner state (see vsg.h).")
       .def("synchronize", [](GpuBruteForceIndex& self) {
         py::gil_scoped_release nogil;
         if (vsg_synchronize(self.handle()) != VSG_OK) throw std::runtime_error(vsg_last_error());
@@ -217,6 +227,7 @@ PYBIND11_MODULE(_vecsearch, m) {
       })
       .def("__len__", &GpuBruteForceIndex::size);
 
+  m.attr("ALL_ROWS") = py::int_(static_cast<unsigned long long>(VSG_ALL_ROWS));
   m.def("cuda_available", [] { return cuda_available(); },
         "True if a GPU is visible (API-compatible name; the runtime is HIP).");
   m.def("hip_available", [] { return hip_available(); }, "True if a HIP GPU is visible.");

@@ -11,7 +11,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <set>
+#include <tuple>
+#include <utility>
 #include <new>
 #include <string>
 #include <vector>
@@ -106,6 +110,10 @@ struct vsg_index {
   float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
   int scan_waves_per_cu = 8;
   int auto_short_batch = 32;
+  vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
+  bool lists_autotune = true;
+  std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class
+  vsg::ListsTiles lists_last;   // tile of the last filtered call
 
   // low-precision copies for the shortlist (index 1 = bf16, 2 = fp8), rebuilt lazily
   struct Prec {
@@ -284,6 +292,33 @@ struct TimedScope {
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
     }
+  }
+};
+
+// TimingSink over the index's event list (for kernels launched outside this file).
+struct IndexTiming : vsg::TimingSink {
+  vsg_index* h;
+  hipEvent_t a = nullptr;
+  explicit IndexTiming(vsg_index* h_) : h(h_) {}
+  void begin(hipStream_t st) override {
+    a = nullptr;
+    if (!h->timing) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventRecord(a, st) != hipSuccess) {
+      if (a) (void)hipEventDestroy(a);
+      a = nullptr;
+      h->timing = false;
+    }
+  }
+  void end(hipStream_t st, int kind, uint32_t launches) override {
+    if (!a) return;
+    hipEvent_t b = nullptr;
+    if (h->timing && hipEventCreate(&b) == hipSuccess && hipEventRecord(b, st) == hipSuccess) {
+      h->timed.push_back({a, b, kind, launches});
+    } else {
+      (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+    a = nullptr;
   }
 };
 
@@ -672,9 +707,10 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
         range[2 * i + 1] = allowed;
       }
       std::vector<uint32_t> ks(nq, uint32_t(k));
+      IndexTiming tm(h);
       return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
                                     h->stream, queries, nq, ks.data(), k, range.data(), ids.data(),
-                                    ids.size(), out_ids, out_dists, g_err);
+                                    ids.size(), out_ids, out_dists, g_err, &tm, h->lists_tiles, &h->lists_last);
     }
   }
   hipError_t e;
@@ -718,6 +754,47 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
   return enqueue_search(h, d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
+// ---- tile autotuner of the filtered scan ---------------------------------------------------
+// A call's shape class is (dim, k > 64, log2 queries per distinct range, log2 rows per distinct
+// range). The first call of a class that is large enough to time (>= 2^24 query-row pairs) runs
+// once per candidate tile, each timed on the scan kernel alone, and the class keeps the fastest.
+// Every candidate returns the same bits, so the outputs of that call are valid whichever ran
+// last.
+
+// Scan-kernel stopwatch for the tuner (search_lists_host synchronises before returning).
+struct ScanStopwatch : vsg::TimingSink {
+  hipEvent_t pending = nullptr, a = nullptr, b = nullptr;
+  ~ScanStopwatch() override {
+    for (hipEvent_t e : {pending, a, b})
+      if (e) (void)hipEventDestroy(e);
+  }
+  void begin(hipStream_t st) override {
+    if (!pending && hipEventCreate(&pending) != hipSuccess) pending = nullptr;
+    if (pending) (void)hipEventRecord(pending, st);
+  }
+  void end(hipStream_t st, int kind, uint32_t) override {
+    if (kind != VSG_KERNEL_LISTS || !pending) return;
+    if (!b && hipEventCreate(&b) != hipSuccess) b = nullptr;
+    if (!b) return;
+    std::swap(a, pending);
+    (void)hipEventRecord(b, st);
+  }
+  float ms() {
+    float v = -1.f;
+    if (a && b && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&v, a, b) == hipSuccess) return v;
+    return -1.f;
+  }
+};
+
+static int ilog2_floor(uint64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
 int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
                      size_t kmax, const uint64_t* range, const uint32_t* allow_ids, size_t n_ids,
                      uint32_t* out_ids, float* out_dists) {
@@ -729,9 +806,79 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
     return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
   std::lock_guard<std::mutex> g(h->mu);
   VSG_HIP(hipSetDevice(h->device));
+  IndexTiming tm(h);
+  const vsg::ListsTiles& fixed = h->lists_tiles;
+  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0;
+  if (planner && h->lists_autotune) {
+    std::set<std::pair<uint64_t, uint64_t>> distinct;
+    uint64_t pairs = 0, rows = 0;
+    for (size_t i = 0; i < nq; ++i) {
+      const bool all = range[2 * i] == VSG_ALL_ROWS;
+      const uint64_t len = all ? h->n : (range[2 * i + 1] >= range[2 * i] ? range[2 * i + 1] - range[2 * i] : 0);
+      pairs += len;
+      const std::pair<uint64_t, uint64_t> rk(all ? uint64_t(VSG_ALL_ROWS) : range[2 * i],
+                                             all ? uint64_t(0) : range[2 * i + 1]);
+      if (distinct.insert(rk).second)
+        rows += len;
+    }
+    const size_t nd = std::max<size_t>(1, distinct.size());
+    const auto key = std::make_tuple(h->dim, kmax > 64 ? 1 : 0, ilog2_floor(nq / nd + 1),
+                                     ilog2_floor(rows / nd + 1));
+    auto it = h->lists_tuned.find(key);
+    if (it != h->lists_tuned.end()) {
+      return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
+                                    h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids,
+                                    out_ids, out_dists, g_err, &tm, it->second, &h->lists_last);
+    }
+    if (pairs >= (uint64_t(1) << 24)) {
+      static const int cand[][3] = {{8, 2, 0},    {4, 4, 0},    {4, 4, 1024}, {4, 4, 4096},
+                                    {8, 2, 1024}, {2, 4, 0},    {4, 2, 1024}, {8, 4, 4096}};
+      vsg::ListsTiles best;
+      float best_ms = -1.f;
+      for (const auto& c : cand) {
+        vsg::ListsTiles t;
+        t.qb = c[0];
+        t.r = c[1];
+        t.chunk = uint64_t(c[2]);
+        ScanStopwatch sw;
+        vsg::ListsTiles used;
+        const int rc = vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric,
+                                              h->num_cus, h->stream, queries, nq, ks, kmax, range,
+                                              allow_ids, n_ids, out_ids, out_dists, g_err, &sw, t, &used);
+        if (rc) return rc;
+        const float ms = sw.ms();
+        if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
+          best_ms = ms;
+          best = t;
+          h->lists_last = used;
+        }
+      }
+      if (best_ms >= 0.f) h->lists_tuned[key] = best;
+      return VSG_OK;
+    }
+  }
   return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
                                 h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids,
-                                out_ids, out_dists, g_err);
+                                out_ids, out_dists, g_err, &tm, fixed, &h->lists_last);
+}
+
+int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
+  if (!h || !name || !value) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(h->mu);
+  const std::string key(name);
+  if (key == "scan_waves_per_cu") *value = h->scan_waves_per_cu;
+  else if (key == "auto_short_batch") *value = h->auto_short_batch;
+  else if (key == "lists_qb") *value = h->lists_tiles.qb;
+  else if (key == "lists_r") *value = h->lists_tiles.r;
+  else if (key == "lists_chunk") *value = int64_t(h->lists_tiles.chunk);
+  else if (key == "lists_blocks_per_cu") *value = h->lists_tiles.blocks_per_cu;
+  else if (key == "lists_autotune") *value = h->lists_autotune ? 1 : 0;
+  else if (key == "lists_tuned_classes") *value = int64_t(h->lists_tuned.size());
+  else if (key == "lists_last_qb") *value = h->lists_last.qb;
+  else if (key == "lists_last_r") *value = h->lists_last.r;
+  else if (key == "lists_last_chunk") *value = int64_t(h->lists_last.chunk);
+  else return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
+  return VSG_OK;
 }
 
 int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
@@ -746,6 +893,35 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
   if (key == "auto_short_batch") {
     if (value < 1) return fail(VSG_ERR_INVALID_ARGUMENT, "auto_short_batch must be >= 1");
     h->auto_short_batch = int(value);
+    return VSG_OK;
+  }
+  if (key == "lists_qb") {
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_qb must be 0 (auto), 1, 2, 4, 8 or 16");
+    h->lists_tiles.qb = int(value);
+    return VSG_OK;
+  }
+  if (key == "lists_r") {
+    if (value != 0 && value != 2 && value != 4)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 2 or 4");
+    h->lists_tiles.r = int(value);
+    return VSG_OK;
+  }
+  if (key == "lists_chunk") {
+    if (value < 0 || value > (int64_t(1) << 24))
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_chunk must be 0 (auto) or at most 2^24 rows");
+    h->lists_tiles.chunk = uint64_t(value);
+    return VSG_OK;
+  }
+  if (key == "lists_autotune") {
+    if (value != 0 && value != 1) return fail(VSG_ERR_INVALID_ARGUMENT, "lists_autotune must be 0 or 1");
+    h->lists_autotune = value != 0;
+    if (!h->lists_autotune) h->lists_tuned.clear();
+    return VSG_OK;
+  }
+  if (key == "lists_blocks_per_cu") {
+    if (value < 0 || value > 16) return fail(VSG_ERR_INVALID_ARGUMENT, "lists_blocks_per_cu must be in 0..16");
+    h->lists_tiles.blocks_per_cu = int(value);
     return VSG_OK;
   }
   return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);

@@ -84,10 +84,28 @@ struct ListsWorkspace {
   void release();
 };
 
+// Tile shape of the filtered scan (0 = chosen by the planner): queries per work item (qb),
+// row groups of 16 per lane step (r), allowed rows per item (chunk), blocks per CU.
+struct ListsTiles {
+  int qb = 0, r = 0;
+  uint64_t chunk = 0;
+  int blocks_per_cu = 0;
+};
+// Supported (qb, r) pairs of the filtered scan kernel.
+bool lists_tile_supported(int qb, int r);
+
+// Event timing of a group of launches (implemented by the index when kernel timing is on).
+struct TimingSink {
+  virtual ~TimingSink() = default;
+  virtual void begin(hipStream_t st) = 0;
+  virtual void end(hipStream_t st, int kind, uint32_t launches) = 0;
+};
+
 int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t dim, size_t dp,
                       int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
                       const uint32_t* ks, size_t kmax, const uint64_t* range,
                       const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
-                      float* out_dists, std::string& err);
+                      float* out_dists, std::string& err, TimingSink* timing,
+                      const ListsTiles& tiles, ListsTiles* used);
 
 }  // namespace vsg

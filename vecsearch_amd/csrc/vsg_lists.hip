@@ -28,10 +28,11 @@
 namespace vsg {
 
 struct ListItem {
-  uint32_t lb, le;  // range of allow_ids
+  uint32_t lb, le;  // range of allow_ids (direct: range of rows)
   uint32_t chunk;   // chunk index within the group
   uint32_t qoff;    // first entry in qperm
   uint32_t nq;      // queries in this block
+  uint32_t direct;  // 1: unfiltered group, position p is row p
 };
 
 template <int QB, int R, int D16, int KP, bool L2>
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
       for (int j = 0; j < R; ++j) {
         const uint32_t idx = r0 + j * 16 + g;
         ok[j] = idx < w.le;
-        row[j] = allow_ids[ok[j] ? idx : w.lb];
+        row[j] = w.direct ? (ok[j] ? idx : w.lb) : allow_ids[ok[j] ? idx : w.lb];
         xr[j] = base + size_t(row[j]) * dp;
       }
       float dist[R][QB];
@@ -159,6 +160,11 @@ struct ListsArgs {
 
 template <int QB, int R, int D16, int KP, bool L2>
 hipError_t launch_lists_t(const ListsArgs& a) {
+  if (a.lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lists_scan_kernel<QB, R, D16, KP, L2>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(a.lds));
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((lists_scan_kernel<QB, R, D16, KP, L2>), a.grid, dim3(kBlock), a.lds, a.st,
                      a.base, a.dp, a.d, a.q, a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off,
                      a.L, a.part, a.counter);
@@ -181,6 +187,17 @@ hipError_t launch_lists_d(bool l2, int kp, const ListsArgs& a) {
   }
 }
 
+hipError_t launch_lists_tile(int qb, int r, bool l2, int kp, const ListsArgs& a) {
+  if (qb == 16 && r == 2) return launch_lists_d<16, 2>(l2, kp, a);
+  if (qb == 8 && r == 4) return launch_lists_d<8, 4>(l2, kp, a);
+  if (qb == 8 && r == 2) return launch_lists_d<8, 2>(l2, kp, a);
+  if (qb == 4 && r == 4) return launch_lists_d<4, 4>(l2, kp, a);
+  if (qb == 4 && r == 2) return launch_lists_d<4, 2>(l2, kp, a);
+  if (qb == 2 && r == 4) return launch_lists_d<2, 4>(l2, kp, a);
+  if (qb == 1 && r == 4) return launch_lists_d<1, 4>(l2, kp, a);
+  return hipErrorInvalidValue;
+}
+
 std::string hip_msg(const char* what, hipError_t e) {
   return std::string(what) + ": " + hipGetErrorString(e);
 }
@@ -190,6 +207,11 @@ int hip_status(hipError_t e) {
 }
 
 }  // namespace
+
+bool lists_tile_supported(int qb, int r) {
+  return (qb == 16 && r == 2) || (qb == 8 && (r == 2 || r == 4)) || (qb == 4 && (r == 2 || r == 4)) ||
+         (qb == 2 && r == 4) || (qb == 1 && r == 4);
+}
 
 void ListsWorkspace::release() {
   for (int i = 0; i < 8; ++i) {
@@ -219,10 +241,18 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
                       int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
                       const uint32_t* ks, size_t kmax, const uint64_t* range,
                       const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
-                      float* out_dists, std::string& err) {
+                      float* out_dists, std::string& err, TimingSink* timing,
+                      const ListsTiles& tiles, ListsTiles* used) {
   // ---- validate (hnsw.cpp:326: an empty allowed set returns nothing) ----
   const uint64_t total = n_ids;
   for (size_t i = 0; i < nq; ++i) {
+    if (range[2 * i] == VSG_ALL_ROWS) {
+      if (ks[i] > kmax) {
+        err = "ks[i] must be <= kmax";
+        return VSG_ERR_INVALID_ARGUMENT;
+      }
+      continue;
+    }
     if (range[2 * i + 1] < range[2 * i] || range[2 * i + 1] > n_ids) {
       err = "each query's range must satisfy begin <= end <= len(allow_ids)";
       return VSG_ERR_INVALID_ARGUMENT;
@@ -237,9 +267,19 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     return VSG_ERR_INVALID_ARGUMENT;
   }
   // Group queries by identical range; each group's ids must be increasing and < n.
+  // (unfiltered queries form the group keyed (VSG_ALL_ROWS, 0))
   std::map<std::pair<uint64_t, uint64_t>, std::vector<uint32_t>> groups;
-  for (size_t i = 0; i < nq; ++i) groups[{range[2 * i], range[2 * i + 1]}].push_back(uint32_t(i));
+  for (size_t i = 0; i < nq; ++i) {
+    if (range[2 * i] == VSG_ALL_ROWS)
+      groups[{VSG_ALL_ROWS, 0}].push_back(uint32_t(i));
+    else
+      groups[{range[2 * i], range[2 * i + 1]}].push_back(uint32_t(i));
+  }
+  auto span_of = [&](const std::pair<uint64_t, uint64_t>& key) {
+    return key.first == VSG_ALL_ROWS ? std::make_pair(uint64_t(0), uint64_t(n)) : key;
+  };
   for (const auto& kv : groups) {
+    if (kv.first.first == VSG_ALL_ROWS) continue;
     const uint64_t lb = kv.first.first, le = kv.first.second;
     for (uint64_t p = lb; p < le; ++p) {
       if (allow_ids[p] >= n || (p > lb && allow_ids[p] <= allow_ids[p - 1])) {
@@ -254,48 +294,68 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   }
 
   // ---- plan: chunks and work items ----
-  // queries per item: 4 waves x qb x dp floats of LDS must fit in 64 KB
-  int qb = 8;
-  while (qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > 65536) qb /= 2;
-  if (size_t(kWavesPerBlock) * qb * dp * sizeof(float) > 65536) {
+  // queries per item: 4 waves x qb x dp floats of LDS must fit in 64 KB (160 KB for a
+  // requested tile)
+  int qb = tiles.qb ? tiles.qb : 8, r = tiles.r;
+  const size_t lds_cap = tiles.qb ? size_t(160) << 10 : size_t(65536);
+  while (qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) qb /= 2;
+  if (size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) {
     err = "dim too large for the filtered scan (at most 4096)";
     return VSG_ERR_INVALID_ARGUMENT;
   }
+  if (!r || !lists_tile_supported(qb, r)) r = qb >= 4 ? 2 : 4;
+  if (qb == 16 && r != 2) r = 2;
   // chunk rows: aim for ~8 items per wave slot, within [256, 8192], and at most
   // kMaxListsPerQuery chunks per query
   uint64_t visits = 0;
-  for (const auto& kv : groups) visits += (kv.first.second - kv.first.first) * ((kv.second.size() + qb - 1) / qb);
+  for (const auto& kv : groups) {
+    const auto sp = span_of(kv.first);
+    visits += (sp.second - sp.first) * ((kv.second.size() + qb - 1) / qb);
+  }
   const uint64_t wave_slots = uint64_t(num_cus) * 16;
   uint64_t C = visits / (wave_slots * 8) + 1;
   C = std::min<uint64_t>(8192, std::max<uint64_t>(256, (C + 63) / 64 * 64));
+  if (tiles.chunk) C = std::max<uint64_t>(64, (tiles.chunk + 63) / 64 * 64);
   uint64_t max_len = 0;
-  for (const auto& kv : groups) max_len = std::max(max_len, kv.first.second - kv.first.first);
+  for (const auto& kv : groups) {
+    const auto sp = span_of(kv.first);
+    max_len = std::max(max_len, sp.second - sp.first);
+  }
   while ((max_len + C - 1) / C > kMaxListsPerQuery) C *= 2;
 
+  if (used) {
+    used->qb = qb;
+    used->r = r;
+    used->chunk = C;
+    used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : 4;
+  }
   std::vector<ListItem> items;
   std::vector<uint32_t> qperm;
   qperm.reserve(nq);
   std::vector<uint64_t> slot_off(nq + 1, 0);
   std::vector<uint32_t> nchunks_of(nq, 0);
+  // Items run chunk-major within a group: the query blocks of one chunk are pulled one after
+  // another, so the chunk's rows are re-read from cache rather than from HBM.
   for (const auto& kv : groups) {
-    const uint64_t lb = kv.first.first, le = kv.first.second;
+    const auto sp = span_of(kv.first);
+    const uint64_t lb = sp.first, le = sp.second;
+    const uint32_t direct = kv.first.first == VSG_ALL_ROWS ? 1u : 0u;
     const uint32_t nch = uint32_t((le - lb + C - 1) / C);
     const auto& qs = kv.second;
     for (uint32_t q : qs) nchunks_of[q] = nch;
-    for (size_t b0 = 0; b0 < qs.size(); b0 += qb) {
-      const uint32_t qoff = uint32_t(qperm.size());
-      const uint32_t nb = uint32_t(std::min<size_t>(qb, qs.size() - b0));
-      for (uint32_t j = 0; j < nb; ++j) qperm.push_back(qs[b0 + j]);
-      for (uint32_t c = 0; c < nch; ++c) {
+    const uint32_t qoff0 = uint32_t(qperm.size());
+    for (uint32_t q : qs) qperm.push_back(q);
+    for (uint32_t c = 0; c < nch; ++c)
+      for (size_t b0 = 0; b0 < qs.size(); b0 += qb) {
         ListItem it;
         it.lb = uint32_t(lb + uint64_t(c) * C);
         it.le = uint32_t(std::min<uint64_t>(le, lb + uint64_t(c + 1) * C));
         it.chunk = c;
-        it.qoff = qoff;
-        it.nq = nb;
+        it.qoff = qoff0 + uint32_t(b0);
+        it.nq = uint32_t(std::min<size_t>(qb, qs.size() - b0));
+        it.direct = direct;
         items.push_back(it);
       }
-    }
   }
   for (size_t i = 0; i < nq; ++i) slot_off[i + 1] = slot_off[i] + nchunks_of[i];
   const uint32_t L = uint32_t(kmax);
@@ -334,7 +394,8 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   if (!items.empty()) {
     ListsArgs a;
     const uint64_t want_blocks = (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
-    a.grid = dim3(unsigned(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * 4)));
+    const int bpc = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : 4;
+    a.grid = dim3(unsigned(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * bpc)));
     a.lds = size_t(kWavesPerBlock) * qb * dp * sizeof(float);
     a.st = st;
     a.base = base;
@@ -352,16 +413,16 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     a.counter = ws.counter;
     const bool l2 = metric == VSG_METRIC_L2;
     const int kp = kmax <= 64 ? 64 : 128;
-    hipError_t e = qb == 8   ? launch_lists_d<8, 2>(l2, kp, a)
-                   : qb == 4 ? launch_lists_d<4, 2>(l2, kp, a)
-                   : qb == 2 ? launch_lists_d<2, 4>(l2, kp, a)
-                             : launch_lists_d<1, 4>(l2, kp, a);
-    LISTS_HIP(e);
+    if (timing) timing->begin(st);
+    LISTS_HIP(launch_lists_tile(qb, r, l2, kp, a));
+    if (timing) timing->end(st, VSG_KERNEL_LISTS, 1);
   }
+  if (timing) timing->begin(st);
   LISTS_HIP(launch_merge(uint32_t(nq), static_cast<const uint64_t*>(ws.dev[kDPart]),
                          static_cast<const uint64_t*>(ws.dev[kDSlot]), 0, L,
                          static_cast<const uint32_t*>(ws.dev[kDKs]), 0, nullptr, d_ids, d_dists,
                          uint32_t(kmax), st));
+  if (timing) timing->end(st, VSG_KERNEL_MERGE, 1);
   LISTS_HIP(hipMemcpyAsync(ws.host[1], d_ids, nq * kmax * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   LISTS_HIP(hipMemcpyAsync(ws.host[2], d_dists, nq * kmax * sizeof(float), hipMemcpyDeviceToHost, st));
   LISTS_HIP(hipStreamSynchronize(st));
