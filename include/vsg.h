@@ -149,6 +149,49 @@ int vsg_synchronize(vsg_index* h);
  * for measurement code that needs the exact HBM footprint. */
 int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim);
 
+/* ---- serving loop (no counterpart in the reference: its GPU index serialises every call
+ * behind one mutex, gpu_brute_force.cu:351,430) ----
+ * Requests (a block of queries, one k, optionally an allowed-id list) are queued by
+ * vsg_server_submit and answered by continuous batching: each of `slots` worker threads owns a
+ * HIP stream and a private workspace, takes as many queued requests as fit `max_batch_queries`
+ * and the admission budget, runs them as one device batch, and completes them; requests that
+ * arrive meanwhile join the next free slot's batch, so batches overlap on the device. A request
+ * is admitted only while the estimated workspace bytes of the requests in flight stay within
+ * `workspace_bytes` (the first request of an idle server is always admitted). Inside a batch,
+ * filtered requests (and unfiltered ones when fewer than the index's auto_short_batch) run in
+ * one persistent launch of the filtered scan; larger unfiltered groups take `method`. Results
+ * equal the index's own search of the same request (bit-identical), alone or in any mix.
+ * The caller keeps queries, allow_ids and the output buffers alive until the request is
+ * waited for. While a server exists, vsg_add / vsg_reserve wait for in-flight batches. */
+typedef struct vsg_server vsg_server;
+typedef struct {
+  size_t max_batch_queries; /* queries per device batch (0: 4096) */
+  size_t workspace_bytes;   /* admission budget of requests in flight (0: 4 GiB) */
+  int slots;                /* batches in flight, one stream each (0: 2) */
+  int method;               /* VSG_METHOD_* for unfiltered queries (default AUTO) */
+} vsg_server_config;
+
+typedef struct {
+  uint64_t requests, queries, batches; /* completed */
+  uint64_t admission_waits;            /* times a free slot waited for workspace budget */
+  uint64_t peak_inflight_bytes;        /* largest admitted estimate in flight */
+  uint64_t max_batch_requests;         /* most requests in one batch */
+} vsg_server_stats;
+
+int vsg_server_create(vsg_index* h, const vsg_server_config* cfg, vsg_server** out);
+/* allow_ids == NULL: unfiltered; else the request may return only rows allow_ids[0 .. n_allowed)
+ * (strictly increasing, < size(); n_allowed == 0 returns nothing). k <= VSG_MAX_K.
+ * Outputs: nq x k ids / distances. *ticket identifies the request for vsg_server_wait. */
+int vsg_server_submit(vsg_server* s, const float* queries, size_t nq, size_t k,
+                      const uint32_t* allow_ids, size_t n_allowed, uint32_t* out_ids,
+                      float* out_dists, uint64_t* ticket);
+/* Blocks until the request is done; returns its status (VSG_OK or the error of its batch) and,
+ * if latency_ms is not NULL, the time from submit to completion. A ticket is waited for once. */
+int vsg_server_wait(vsg_server* s, uint64_t ticket, double* latency_ms);
+int vsg_server_get_stats(vsg_server* s, vsg_server_stats* out);
+/* Finishes every queued request, then stops the workers. */
+void vsg_server_destroy(vsg_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -258,7 +258,7 @@ def test_shortlist_matches_oracle(mixture_70k, method, k):
     assert_same(got, exp)
 
 
-@pytest.mark.parametrize("d", [100, 128, 256])
+@pytest.mark.parametrize("d", [100, 128, 256, 768])
 def test_shortlist_other_dims(d):
     x = dg.mixture(66_000, d, seed=42)
     q = dg.mixture(70, d, seed=43)

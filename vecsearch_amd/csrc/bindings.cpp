@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
+#include <utility>
 
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -69,6 +71,101 @@ py::array_t<int64_t> ids_to_int64(const py::array_t<uint32_t>& ids) {
     dst[i] = src[i] == 0xFFFFFFFFu ? -1 : int64_t(src[i]);
   return out;
 }
+
+// Python face of the serving loop (vsg_server_*): submit() keeps the request's arrays alive
+// until wait() returns them.
+class PyServer {
+ public:
+  PyServer(py::object index, size_t max_batch_queries, size_t workspace_bytes, int slots,
+           const std::string& method)
+      : index_(std::move(index)) {
+    auto& idx = index_.cast<GpuBruteForceIndex&>();
+    dim_ = idx.dim();
+    vsg_server_config cfg{};
+    cfg.max_batch_queries = max_batch_queries;
+    cfg.workspace_bytes = workspace_bytes;
+    cfg.slots = slots;
+    cfg.method = static_cast<int>(parse_method(method));
+    if (vsg_server_create(idx.handle(), &cfg, &s_) != VSG_OK) throw std::invalid_argument(vsg_last_error());
+  }
+  ~PyServer() { close(); }
+
+  uint64_t submit(FloatArray q, size_t k, const py::object& allow) {
+    if (!s_) throw std::runtime_error("server is closed");
+    if (q.ndim() == 1) q = q.reshape({py::ssize_t(1), q.shape(0)});
+    check_2d(q, dim_);
+    Pending p;
+    p.q = q;
+    const size_t nq = q.shape(0);
+    const uint32_t* ap = nullptr;
+    size_t na = 0;
+    if (!allow.is_none()) {
+      p.allow = py::array_t<uint32_t, py::array::c_style | py::array::forcecast>::ensure(allow);
+      if (!p.allow || p.allow.ndim() != 1) throw std::invalid_argument("allow_ids must be a 1-D array of ids");
+      ap = p.allow.data();
+      na = size_t(p.allow.shape(0));
+      static const uint32_t kEmpty = 0;
+      if (na == 0) ap = &kEmpty;  // an empty allowed set is still a filter
+    }
+    p.ids = py::array_t<uint32_t>({nq, k});
+    p.dists = py::array_t<float>({nq, k});
+    uint64_t t = 0;
+    if (vsg_server_submit(s_, p.q.data(), nq, k, ap, na, p.ids.mutable_data(), p.dists.mutable_data(), &t) != VSG_OK)
+      throw std::invalid_argument(vsg_last_error());
+    pending_.emplace(t, std::move(p));
+    return t;
+  }
+
+  py::tuple wait(uint64_t ticket) {
+    auto it = pending_.find(ticket);
+    if (it == pending_.end()) throw std::invalid_argument("unknown or already waited ticket");
+    Pending p = std::move(it->second);
+    pending_.erase(it);
+    double ms = 0;
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = vsg_server_wait(s_, ticket, &ms);
+    }
+    if (rc != VSG_OK) throw std::runtime_error(vsg_last_error());
+    return py::make_tuple(ids_to_int64(p.ids), p.dists, ms);
+  }
+
+  py::dict stats() {
+    vsg_server_stats st{};
+    if (!s_ || vsg_server_get_stats(s_, &st) != VSG_OK) throw std::runtime_error("server is closed");
+    py::dict d;
+    d["requests"] = st.requests;
+    d["queries"] = st.queries;
+    d["batches"] = st.batches;
+    d["admission_waits"] = st.admission_waits;
+    d["peak_inflight_bytes"] = st.peak_inflight_bytes;
+    d["max_batch_requests"] = st.max_batch_requests;
+    return d;
+  }
+
+  void close() {
+    if (!s_) return;
+    {
+      py::gil_scoped_release nogil;
+      vsg_server_destroy(s_);
+    }
+    s_ = nullptr;
+    pending_.clear();
+  }
+
+ private:
+  struct Pending {
+    FloatArray q;
+    py::array_t<uint32_t, py::array::c_style | py::array::forcecast> allow;
+    py::array_t<uint32_t> ids;
+    py::array_t<float> dists;
+  };
+  py::object index_;
+  size_t dim_ = 0;
+  vsg_server* s_ = nullptr;
+  std::unordered_map<uint64_t, Pending> pending_;
+};
 
 }  // namespace
 
@@ -226,6 +323,21 @@ PYBIND11_MODULE(_vecsearch, m) {
         return self.metric() == Metric::L2 ? "l2" : "ip";
       })
       .def("__len__", &GpuBruteForceIndex::size);
+
+  py::class_<PyServer>(m, "Server")
+      .def(py::init<py::object, size_t, size_t, int, const std::string&>(), py::arg("index"),
+           py::arg("max_batch_queries") = 4096, py::arg("workspace_bytes") = size_t(4) << 30,
+           py::arg("slots") = 2, py::arg("method") = "auto",
+           "Continuous-batching request loop over an index (see vsg.h, vsg_server_*).")
+      .def("submit", &PyServer::submit, py::arg("queries"), py::arg("k") = 10,
+           py::arg("allow_ids") = py::none(),
+           "Queue a request; returns its ticket. allow_ids: increasing ids the request may return.")
+      .def("wait", &PyServer::wait, py::arg("ticket"),
+           "Block until the request is done: (ids[int64, nq x k], distances[float32], latency_ms).")
+      .def("stats", &PyServer::stats)
+      .def("close", &PyServer::close)
+      .def("__enter__", [](PyServer& s) -> PyServer& { return s; })
+      .def("__exit__", [](PyServer& s, py::object, py::object, py::object) { s.close(); });
 
   m.attr("ALL_ROWS") = py::int_(static_cast<unsigned long long>(VSG_ALL_ROWS));
   m.def("cuda_available", [] { return cuda_available(); },

@@ -12,7 +12,15 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
 #include <mutex>
+#include <shared_mutex>
 #include <set>
 #include <tuple>
 #include <utility>
@@ -100,32 +108,13 @@ constexpr size_t kPartBudgetBytes = size_t(256) << 20;  // per-round candidate l
 
 }  // namespace
 
-struct vsg_index {
-  size_t dim = 0, dp = 0;
-  int metric = VSG_METRIC_L2;
-  int device = 0;
-  int num_cus = 256;
-  size_t n = 0, cap = 0;
-  float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
-  float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
-  int scan_waves_per_cu = 8;
-  int auto_short_batch = 32;
-  vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
-  bool lists_autotune = true;
-  std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class
-  vsg::ListsTiles lists_last;   // tile of the last filtered call
+namespace {
 
-  // low-precision copies for the shortlist (index 1 = bf16, 2 = fp8), rebuilt lazily
-  struct Prec {
-    void* frag = nullptr;             // fragment layout, cap rows
-    float* scale = nullptr;           // fp8 per-row scale
-    double* norms = nullptr;          // per row |x|^2, |x^|^2, |x - x^|^2
-    unsigned long long* maxes = nullptr;  // max |x|, |x^|, |x - x^| (fp64 bits)
-    size_t rows = 0, cap = 0;
-  } prec[3];
-  int ks = 0;  // 32-dim k-steps of the copies (0: shortlist unsupported for this dim)
-
-  // shortlist workspace
+// Per-search scratch: device workspace, pinned staging and statistics. The index owns one
+// for its host API (calls serialised by the index mutex); each serving slot owns another, so
+// batches in flight on different streams never share scratch memory.
+struct Workspace {
+  // shortlist
   DevBuf<uint8_t> s_qfrag;
   DevBuf<float> s_qscale, s_E, s_thr, s_qaux;
   DevBuf<double> s_qnorms;
@@ -136,9 +125,7 @@ struct vsg_index {
   HostBuf<uint32_t> s_hover;
   uint64_t st_emitted = 0, st_rescored = 0, st_fallback = 0;
   size_t st_nq = 0;
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-
+  // exact scan and host staging
   DevBuf<float> q;
   DevBuf<uint64_t> part;
   DevBuf<uint32_t> ids;
@@ -148,30 +135,9 @@ struct vsg_index {
   HostBuf<uint32_t> hids;
   HostBuf<float> hdists;
   vsg::ListsWorkspace lists;
+  vsg::ListsTiles lists_last;  // tile of the last filtered call
 
-  // kernel timing (vsg_kernel_timing)
-  struct Timed {
-    hipEvent_t a, b;
-    int kind;  // VSG_KERNEL_* (vsg.h)
-    uint32_t launches;
-  };
-  bool timing = false;
-  std::vector<Timed> timed;
-
-  ~vsg_index() {
-    for (auto& t : timed) {
-      (void)hipEventDestroy(t.a);
-      (void)hipEventDestroy(t.b);
-    }
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (base) (void)hipFree(base);
-    if (tiles) (void)hipFree(tiles);
-    for (auto& p : prec) {
-      if (p.frag) (void)hipFree(p.frag);
-      if (p.scale) (void)hipFree(p.scale);
-      if (p.norms) (void)hipFree(p.norms);
-      if (p.maxes) (void)hipFree(p.maxes);
-    }
+  void release() {
     s_qfrag.release();
     s_qscale.release();
     s_E.release();
@@ -197,6 +163,71 @@ struct vsg_index {
     hids.release();
     hdists.release();
     lists.release();
+  }
+};
+
+}  // namespace
+
+struct vsg_index {
+  size_t dim = 0, dp = 0;
+  int metric = VSG_METRIC_L2;
+  int device = 0;
+  int num_cus = 256;
+  size_t n = 0, cap = 0;
+  float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
+  float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
+  int scan_waves_per_cu = 8;
+  int auto_short_batch = 32;
+  vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
+  bool lists_autotune = true;
+  std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class
+
+  // low-precision copies for the shortlist (index 1 = bf16, 2 = fp8), built on first use
+  struct Prec {
+    void* frag = nullptr;             // fragment layout, cap rows
+    float* scale = nullptr;           // fp8 per-row scale
+    double* norms = nullptr;          // per row |x|^2, |x^|^2, |x - x^|^2
+    unsigned long long* maxes = nullptr;  // max |x|, |x^|, |x - x^| (fp64 bits)
+    size_t rows = 0, cap = 0;
+  } prec[3];
+  int ks = 0;  // 32-dim k-steps of the copies (0: shortlist unsupported for this dim)
+
+  hipStream_t stream = nullptr;
+  Workspace ws;  // the host API's workspace
+
+  // Locking: `mu` serialises calls of the host API (as the reference's per-index mutex does,
+  // gpu_brute_force.cu:351). Searches hold `rw` shared and writers (add, reserve, set_param)
+  // hold it exclusively, so serving slots search concurrently with the host API but never
+  // with a change of the base. prec_mu guards the lazy low-precision copies, tune_mu the tile
+  // table, timing_mu the event list.
+  std::mutex mu;
+  std::shared_mutex rw;
+  std::mutex prec_mu, tune_mu, timing_mu;
+
+  // kernel timing (vsg_kernel_timing)
+  struct Timed {
+    hipEvent_t a, b;
+    int kind;  // VSG_KERNEL_* (vsg.h)
+    uint32_t launches;
+  };
+  std::atomic<bool> timing{false};
+  std::vector<Timed> timed;
+
+  ~vsg_index() {
+    for (auto& t : timed) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (base) (void)hipFree(base);
+    if (tiles) (void)hipFree(tiles);
+    for (auto& p : prec) {
+      if (p.frag) (void)hipFree(p.frag);
+      if (p.scale) (void)hipFree(p.scale);
+      if (p.norms) (void)hipFree(p.norms);
+      if (p.maxes) (void)hipFree(p.maxes);
+    }
+    ws.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -287,6 +318,7 @@ struct TimedScope {
   ~TimedScope() {
     if (!a || !b) return;
     if (h->timing && hipEventRecord(b, st) == hipSuccess) {
+      std::lock_guard<std::mutex> g(h->timing_mu);
       h->timed.push_back({a, b, kind, launches});
     } else {
       (void)hipEventDestroy(a);
@@ -313,6 +345,7 @@ struct IndexTiming : vsg::TimingSink {
     if (!a) return;
     hipEvent_t b = nullptr;
     if (h->timing && hipEventCreate(&b) == hipSuccess && hipEventRecord(b, st) == hipSuccess) {
+      std::lock_guard<std::mutex> g(h->timing_mu);
       h->timed.push_back({a, b, kind, launches});
     } else {
       (void)hipEventDestroy(a);
@@ -323,14 +356,14 @@ struct IndexTiming : vsg::TimingSink {
 };
 
 // Enqueue an exact streaming search of nq device-resident queries on stream st.
-int enqueue_exact(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+int enqueue_exact(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
                   const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
   const ScanPlan p = plan_scan(h);
   const int qb = queries_per_pass(method, nq, h->dp);
   const size_t per_query = size_t(p.W) * k * sizeof(uint64_t);
   size_t round = std::max<size_t>(qb, kPartBudgetBytes / per_query / qb * qb);
   round = std::min(round, (nq + qb - 1) / qb * qb);
-  hipError_t e = h->part.reserve(round * size_t(p.W) * k);
+  hipError_t e = w.part.reserve(round * size_t(p.W) * k);
   if (e != hipSuccess) return fail_hip("workspace", e);
   const bool l2 = h->metric == VSG_METRIC_L2;
   for (size_t r0 = 0; r0 < nq; r0 += round) {
@@ -342,14 +375,14 @@ int enqueue_exact(vsg_index* h, const float* d_q, size_t nq, size_t k, int metho
         const uint32_t nqp = uint32_t(std::min<size_t>(qb, rq - q0));
         e = vsg::launch_scan(qb, l2, uint32_t(k), p.n_blocks, st, h->tiles, uint32_t(h->n),
                              uint32_t(h->dp), uint32_t(h->dim), qr, uint32_t(q0), nqp,
-                             p.rows_per_wave, d_mask, h->part.p);
+                             p.rows_per_wave, d_mask, w.part.p);
         if (e != hipSuccess) return fail_hip("scan kernel", e);
         ts.launches++;
       }
     }
     {
       TimedScope ts(h, st, VSG_KERNEL_MERGE);
-      e = vsg::launch_merge(uint32_t(rq), h->part.p, nullptr, p.W, uint32_t(k), nullptr,
+      e = vsg::launch_merge(uint32_t(rq), w.part.p, nullptr, p.W, uint32_t(k), nullptr,
                             uint32_t(k), nullptr, d_ids + r0 * k, d_dists + r0 * k, uint32_t(k), st);
       if (e != hipSuccess) return fail_hip("merge kernel", e);
       ts.launches++;
@@ -365,9 +398,12 @@ bool short_supported(const vsg_index* h, size_t k) {
          k <= VSG_MAX_K && size_t(vsg::short_cand_cap()) >= 4 * k;
 }
 
-// Quantized copy `p` (1 bf16, 2 fp8) covering every row of the base.
+// Quantized copy `p` (1 bf16, 2 fp8) covering every row of the base. Built on the index
+// stream, which is synchronised before the copy is used from any other stream.
 int ensure_prec(vsg_index* h, int p) {
+  std::lock_guard<std::mutex> g(h->prec_mu);
   auto& c = h->prec[p];
+  if (c.frag && c.cap == h->cap && c.rows == h->n) return VSG_OK;
   const int ks = vsg::short_ks(uint32_t(h->dim));
   if (c.frag && (c.cap != h->cap || ks != h->ks)) {  // the base grew: rebuild from scratch
     (void)hipFree(c.frag);
@@ -395,23 +431,20 @@ int ensure_prec(vsg_index* h, int p) {
     VSG_HIP(vsg::launch_norm_max(h->stream, c.norms, uint32_t(c.rows), uint32_t(h->n), c.maxes));
     c.rows = h->n;
   }
+  VSG_HIP(hipStreamSynchronize(h->stream));
   return VSG_OK;
 }
 
-int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+int enqueue_search(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
                    const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st);
 
 // Shortlist search of nq device-resident queries (prec 1 bf16, 2 fp8). Synchronises the stream
 // once at the end to learn whether any query overflowed its candidate buffer; those queries are
 // answered again by the exact scan.
-int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, uint32_t* d_ids,
+int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int p, uint32_t* d_ids,
                   float* d_dists, hipStream_t st) {
-  if (st != h->stream) {  // the precision copy is built on the index stream
-    VSG_HIP(hipStreamSynchronize(st));
-  }
   int rc = ensure_prec(h, p);
   if (rc) return rc;
-  if (st != h->stream) VSG_HIP(hipStreamSynchronize(h->stream));
   const auto& c = h->prec[p];
   const int ks = h->ks;
   const uint32_t cap = uint32_t(vsg::short_cand_cap());
@@ -422,29 +455,29 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
   const size_t grp = size_t(vsg::short_group(p, ks)) * 16;  // queries per stage of the MFMA scan
   const size_t qpad = (round + grp - 1) / grp * grp;
   hipError_t e;
-  if ((e = h->s_qfrag.reserve(qpad / 16 * ks * chunk)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_qnorms.reserve(qpad * 3)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_qaux.reserve(qpad / 16 * 32 + vsg::short_aux_pad())) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_tilemin.reserve(round * n_tiles)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_qfrag.reserve(qpad / 16 * ks * chunk)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_qnorms.reserve(qpad * 3)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_qaux.reserve(qpad / 16 * 32 + vsg::short_aux_pad())) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_tilemin.reserve(round * n_tiles)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
   // pass-2 emission regions, one per wave (64 rows): a round of 4,096 queries that each keep a
   // few hundred candidates out of n rows fills a region to a few dozen entries on average;
   // 1,024 leaves a wide margin for clustered data (an overflow sends the round to the exact scan)
   const uint32_t wave_cap = 1024;
   const size_t n_regions = size_t(n_tiles) * vsg::short_waves();
-  if ((e = h->s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_emit_over.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->s_hover.reserve(2)) != hipSuccess) return fail_hip("pinned staging", e);
-  h->st_emitted = h->st_rescored = h->st_fallback = 0;
-  h->st_nq = 0;
+  if ((e = w.s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_emit_over.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_hover.reserve(2)) != hipSuccess) return fail_hip("pinned staging", e);
+  w.st_emitted = w.st_rescored = w.st_fallback = 0;
+  w.st_nq = 0;
 
   std::vector<uint32_t> flags, counts, rescored;
   for (size_t r0 = 0; r0 < nq; r0 += round) {
@@ -453,11 +486,11 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     uint32_t* ids = d_ids + r0 * k;
     float* dists = d_dists + r0 * k;
     VSG_HIP(vsg::launch_quantize(st, qr, uint32_t(h->dim), uint32_t(h->dim), 0, rq, uint32_t(ks), p,
-                                 h->s_qfrag.p, h->s_qscale.p, h->s_qnorms.p));
-    VSG_HIP(vsg::launch_query_bound(st, h->s_qnorms.p, rq, p, uint32_t(32 * ks), c.maxes, h->s_E.p));
-    VSG_HIP(vsg::launch_zero_u32(st, h->s_count.p, rq));
-    VSG_HIP(vsg::launch_zero_u32(st, h->s_nover.p, 1));
-    VSG_HIP(vsg::launch_zero_u32(st, h->s_emit_over.p, 1));
+                                 w.s_qfrag.p, w.s_qscale.p, w.s_qnorms.p));
+    VSG_HIP(vsg::launch_query_bound(st, w.s_qnorms.p, rq, p, uint32_t(32 * ks), c.maxes, w.s_E.p));
+    VSG_HIP(vsg::launch_zero_u32(st, w.s_count.p, rq));
+    VSG_HIP(vsg::launch_zero_u32(st, w.s_nover.p, 1));
+    VSG_HIP(vsg::launch_zero_u32(st, w.s_emit_over.p, 1));
 
     vsg::ShortScanArgs a;
     a.st = st;
@@ -465,34 +498,34 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     a.xfrag = c.frag;
     a.xscale = c.scale;
     a.n = uint32_t(h->n);
-    a.qfrag = h->s_qfrag.p;
-    a.qaux = h->s_qaux.p;
+    a.qfrag = w.s_qfrag.p;
+    a.qaux = w.s_qaux.p;
     a.nq = rq;
     // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk)
     // (a looser bound only grows the candidate set, checked against its capacity)
     const uint32_t tstride = uint32_t(std::min<size_t>(16, std::max<size_t>(1, n_tiles / (4 * k))));
     const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
-    a.tilemin = h->s_tilemin.p;
-    a.emit = h->s_emit.p;
+    a.tilemin = w.s_tilemin.p;
+    a.emit = w.s_emit.p;
     a.wave_cap = wave_cap;
-    a.emit_count = h->s_emit_count.p;
-    a.overflow = h->s_emit_over.p;
-    a.cand = h->s_cand.p;
+    a.emit_count = w.s_emit_count.p;
+    a.overflow = w.s_emit_over.p;
+    a.cand = w.s_cand.p;
     a.cap = cap;
-    a.count = h->s_count.p;
-    VSG_HIP(vsg::launch_query_aux(st, nullptr, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
-                                  h->s_qaux.p));
+    a.count = w.s_count.p;
+    VSG_HIP(vsg::launch_query_aux(st, nullptr, p == 2 ? w.s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
+                                  w.s_qaux.p));
     {
       TimedScope ts(h, st, VSG_KERNEL_SAMPLE);
       a.tstride = tstride;
       a.n_space = a.n_blocks = space1;
       VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
-      VSG_HIP(vsg::launch_select_threshold(st, h->s_tilemin.p, space1, rq, nullptr, uint32_t(k), h->s_E.p,
-                                           h->s_thr.p));
+      VSG_HIP(vsg::launch_select_threshold(st, w.s_tilemin.p, space1, rq, nullptr, uint32_t(k), w.s_E.p,
+                                           w.s_thr.p));
       ts.launches++;
     }
-    VSG_HIP(vsg::launch_query_aux(st, h->s_thr.p, p == 2 ? h->s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
-                                  h->s_qaux.p));
+    VSG_HIP(vsg::launch_query_aux(st, w.s_thr.p, p == 2 ? w.s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
+                                  w.s_qaux.p));
     {
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
       a.tstride = 1;
@@ -507,40 +540,40 @@ int enqueue_short(vsg_index* h, const float* d_q, size_t nq, size_t k, int p, ui
     {
       TimedScope ts(h, st, VSG_KERNEL_RESCORE);
       VSG_HIP(vsg::launch_rescore(st, h->metric == VSG_METRIC_L2, h->base, uint32_t(h->dp), uint32_t(h->dim),
-                                  qr, rq, nullptr, uint32_t(k), h->s_E.p, h->s_cand.p, h->s_count.p, ids,
-                                  dists, uint32_t(k), h->s_flags.p, h->s_nover.p, h->s_rescored.p));
+                                  qr, rq, nullptr, uint32_t(k), w.s_E.p, w.s_cand.p, w.s_count.p, ids,
+                                  dists, uint32_t(k), w.s_flags.p, w.s_nover.p, w.s_rescored.p));
       ts.launches++;
     }
-    VSG_HIP(hipMemcpyAsync(h->s_hover.p, h->s_nover.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    VSG_HIP(hipMemcpyAsync(h->s_hover.p + 1, h->s_emit_over.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VSG_HIP(hipMemcpyAsync(w.s_hover.p, w.s_nover.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VSG_HIP(hipMemcpyAsync(w.s_hover.p + 1, w.s_emit_over.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VSG_HIP(hipStreamSynchronize(st));
-    if (h->s_hover.p[1]) {  // an emission region overflowed: answer this round exactly
-      h->st_fallback += rq;
-      rc = enqueue_exact(h, qr, rq, k, VSG_METHOD_SKINNY, nullptr, ids, dists, st);
+    if (w.s_hover.p[1]) {  // an emission region overflowed: answer this round exactly
+      w.st_fallback += rq;
+      rc = enqueue_exact(h, w, qr, rq, k, VSG_METHOD_SKINNY, nullptr, ids, dists, st);
       if (rc) return rc;
-      h->st_nq += rq;
+      w.st_nq += rq;
       continue;
     }
-    const uint32_t nover = h->s_hover.p[0];
+    const uint32_t nover = w.s_hover.p[0];
     if (h->timing || nover) {
       counts.resize(rq);
       rescored.resize(rq);
-      VSG_HIP(hipMemcpy(counts.data(), h->s_count.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      VSG_HIP(hipMemcpy(rescored.data(), h->s_rescored.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      VSG_HIP(hipMemcpy(counts.data(), w.s_count.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      VSG_HIP(hipMemcpy(rescored.data(), w.s_rescored.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
       for (uint32_t i = 0; i < rq; ++i) {
-        h->st_emitted += counts[i];
-        h->st_rescored += rescored[i];
+        w.st_emitted += counts[i];
+        w.st_rescored += rescored[i];
       }
     }
-    h->st_nq += rq;
+    w.st_nq += rq;
     if (nover) {
       // overflowed queries: exact scan, one query at a time into this round's output rows
       flags.resize(rq);
-      VSG_HIP(hipMemcpy(flags.data(), h->s_flags.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      VSG_HIP(hipMemcpy(flags.data(), w.s_flags.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
       for (uint32_t i = 0; i < rq; ++i) {
         if (!flags[i]) continue;
-        h->st_fallback++;
-        rc = enqueue_search(h, qr + size_t(i) * h->dim, 1, k, VSG_METHOD_SKINNY, nullptr, ids + size_t(i) * k,
+        w.st_fallback++;
+        rc = enqueue_search(h, w, qr + size_t(i) * h->dim, 1, k, VSG_METHOD_SKINNY, nullptr, ids + size_t(i) * k,
                             dists + size_t(i) * k, st);
         if (rc) return rc;
       }
@@ -559,21 +592,133 @@ int check_search_args(const vsg_index* h, size_t k, int method) {
 }
 
 // The method -> path choice (see VSG_METHOD_* in vsg.h).
-int enqueue_search(vsg_index* h, const float* d_q, size_t nq, size_t k, int method,
+int enqueue_search(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
                    const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
   if (d_mask == nullptr && short_supported(h, k)) {
     int p = 0;
     if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16) p = 1;
     else if (method == VSG_METHOD_FP8) p = 2;
     else if (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch)) p = 1;
-    if (p) return enqueue_short(h, d_q, nq, k, p, d_ids, d_dists, st);
+    if (p) return enqueue_short(h, w, d_q, nq, k, p, d_ids, d_dists, st);
   }
-  return enqueue_exact(h, d_q, nq, k, method, d_mask, d_ids, d_dists, st);
+  return enqueue_exact(h, w, d_q, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
 void fill_missing(uint32_t* ids, float* dists, size_t count) {
   std::fill(ids, ids + count, VSG_NONE);
   std::fill(dists, dists + count, INFINITY);
+}
+
+// ---- tile autotuner of the filtered scan ---------------------------------------------------
+// A call's shape class is (dim, k > 64, log2 queries per distinct range, log2 rows per distinct
+// range). The first call of a class that is large enough to time (>= 2^24 query-row pairs) runs
+// once per candidate tile, each timed on the scan kernel alone, and the class keeps the fastest.
+// Every candidate returns the same bits, so the outputs of that call are valid whichever ran
+// last.
+
+// Scan-kernel stopwatch for the tuner (search_lists_host synchronises before returning).
+struct ScanStopwatch : vsg::TimingSink {
+  hipEvent_t pending = nullptr, a = nullptr, b = nullptr;
+  ~ScanStopwatch() override {
+    for (hipEvent_t e : {pending, a, b})
+      if (e) (void)hipEventDestroy(e);
+  }
+  void begin(hipStream_t st) override {
+    if (!pending && hipEventCreate(&pending) != hipSuccess) pending = nullptr;
+    if (pending) (void)hipEventRecord(pending, st);
+  }
+  void end(hipStream_t st, int kind, uint32_t) override {
+    if (kind != VSG_KERNEL_LISTS || !pending) return;
+    if (!b && hipEventCreate(&b) != hipSuccess) b = nullptr;
+    if (!b) return;
+    std::swap(a, pending);
+    (void)hipEventRecord(b, st);
+  }
+  float ms() {
+    float v = -1.f;
+    if (a && b && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&v, a, b) == hipSuccess) return v;
+    return -1.f;
+  }
+};
+
+int ilog2_floor(uint64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
+// Filtered / mixed batch through the persistent lists scan, with the tile autotuner; host
+// buffers in and out, on stream st with workspace w. The caller holds h->rw shared.
+int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, size_t nq,
+              const uint32_t* ks, size_t kmax, const uint64_t* range, const uint32_t* allow_ids,
+              size_t n_ids, uint32_t* out_ids, float* out_dists) {
+  IndexTiming tm(h);
+  const vsg::ListsTiles& fixed = h->lists_tiles;
+  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0;
+  if (planner && h->lists_autotune) {
+    std::set<std::pair<uint64_t, uint64_t>> distinct;
+    uint64_t pairs = 0, rows = 0;
+    for (size_t i = 0; i < nq; ++i) {
+      const bool all = range[2 * i] == VSG_ALL_ROWS;
+      const uint64_t len = all ? h->n : (range[2 * i + 1] >= range[2 * i] ? range[2 * i + 1] - range[2 * i] : 0);
+      pairs += len;
+      const std::pair<uint64_t, uint64_t> rk(all ? uint64_t(VSG_ALL_ROWS) : range[2 * i],
+                                             all ? uint64_t(0) : range[2 * i + 1]);
+      if (distinct.insert(rk).second) rows += len;
+    }
+    const size_t nd = std::max<size_t>(1, distinct.size());
+    const auto key = std::make_tuple(h->dim, kmax > 64 ? 1 : 0, ilog2_floor(nq / nd + 1),
+                                     ilog2_floor(rows / nd + 1));
+    bool have = false;
+    vsg::ListsTiles tuned;
+    {
+      std::lock_guard<std::mutex> g(h->tune_mu);
+      auto it = h->lists_tuned.find(key);
+      if (it != h->lists_tuned.end()) {
+        have = true;
+        tuned = it->second;
+      }
+    }
+    if (have)
+      return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus, st,
+                                    queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists,
+                                    g_err, &tm, tuned, &w.lists_last);
+    if (pairs >= (uint64_t(1) << 24)) {
+      static const int cand[][3] = {{8, 2, 0},    {4, 4, 0},    {4, 4, 1024}, {4, 4, 4096},
+                                    {8, 2, 1024}, {2, 4, 0},    {4, 2, 1024}, {8, 4, 4096}};
+      vsg::ListsTiles best;
+      float best_ms = -1.f;
+      for (const auto& c : cand) {
+        vsg::ListsTiles t;
+        t.qb = c[0];
+        t.r = c[1];
+        t.chunk = uint64_t(c[2]);
+        ScanStopwatch sw;
+        vsg::ListsTiles used;
+        const int rc = vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric,
+                                              h->num_cus, st, queries, nq, ks, kmax, range,
+                                              allow_ids, n_ids, out_ids, out_dists, g_err, &sw, t, &used);
+        if (rc) return rc;
+        const float ms = sw.ms();
+        if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
+          best_ms = ms;
+          best = t;
+          w.lists_last = used;
+        }
+      }
+      if (best_ms >= 0.f) {
+        std::lock_guard<std::mutex> g(h->tune_mu);
+        h->lists_tuned[key] = best;
+      }
+      return VSG_OK;
+    }
+  }
+  return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus, st,
+                                queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists,
+                                g_err, &tm, fixed, &w.lists_last);
 }
 
 }  // namespace
@@ -653,6 +798,7 @@ int vsg_metric(const vsg_index* h, int* metric) {
 int vsg_reserve(vsg_index* h, size_t n) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   std::lock_guard<std::mutex> g(h->mu);
+  std::unique_lock<std::shared_mutex> wl(h->rw);
   if (n >= VSG_NONE) return fail(VSG_ERR_LENGTH, "an index holds at most 2^32 - 2 vectors");
   VSG_HIP(hipSetDevice(h->device));
   return grow(h, n);
@@ -661,6 +807,7 @@ int vsg_reserve(vsg_index* h, size_t n) {
 int vsg_add(vsg_index* h, const float* data, size_t m) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   std::lock_guard<std::mutex> g(h->mu);
+  std::unique_lock<std::shared_mutex> wl(h->rw);
   if (m == 0) return VSG_OK;
   if (!data) return fail(VSG_ERR_INVALID_ARGUMENT, "null data");
   if (h->n + m >= VSG_NONE) return fail(VSG_ERR_LENGTH, "an index holds at most 2^32 - 2 vectors");
@@ -683,6 +830,8 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
   if (nq == 0 || k == 0) return VSG_OK;
   if (!queries || !out_ids || !out_dists) return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
   std::lock_guard<std::mutex> g(h->mu);
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  Workspace& w = h->ws;
   if (h->n == 0) {
     fill_missing(out_ids, out_dists, nq * k);
     return VSG_OK;
@@ -708,33 +857,33 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
       }
       std::vector<uint32_t> ks(nq, uint32_t(k));
       IndexTiming tm(h);
-      return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
+      return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
                                     h->stream, queries, nq, ks.data(), k, range.data(), ids.data(),
-                                    ids.size(), out_ids, out_dists, g_err, &tm, h->lists_tiles, &h->lists_last);
+                                    ids.size(), out_ids, out_dists, g_err, &tm, h->lists_tiles, &w.lists_last);
     }
   }
   hipError_t e;
-  if ((e = h->q.reserve(nq * h->dim)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->ids.reserve(nq * k)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->dists.reserve(nq * k)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = h->hq.reserve(nq * h->dim)) != hipSuccess) return fail_hip("pinned staging", e);
-  if ((e = h->hids.reserve(nq * k)) != hipSuccess) return fail_hip("pinned staging", e);
-  if ((e = h->hdists.reserve(nq * k)) != hipSuccess) return fail_hip("pinned staging", e);
+  if ((e = w.q.reserve(nq * h->dim)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.ids.reserve(nq * k)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.dists.reserve(nq * k)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.hq.reserve(nq * h->dim)) != hipSuccess) return fail_hip("pinned staging", e);
+  if ((e = w.hids.reserve(nq * k)) != hipSuccess) return fail_hip("pinned staging", e);
+  if ((e = w.hdists.reserve(nq * k)) != hipSuccess) return fail_hip("pinned staging", e);
   const uint8_t* d_mask = nullptr;
   if (mask) {
-    if ((e = h->mask.reserve(h->n)) != hipSuccess) return fail_hip("workspace", e);
-    VSG_HIP(hipMemcpyAsync(h->mask.p, mask, h->n, hipMemcpyHostToDevice, h->stream));
-    d_mask = h->mask.p;
+    if ((e = w.mask.reserve(h->n)) != hipSuccess) return fail_hip("workspace", e);
+    VSG_HIP(hipMemcpyAsync(w.mask.p, mask, h->n, hipMemcpyHostToDevice, h->stream));
+    d_mask = w.mask.p;
   }
-  std::memcpy(h->hq.p, queries, nq * h->dim * sizeof(float));
-  VSG_HIP(hipMemcpyAsync(h->q.p, h->hq.p, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  rc = enqueue_search(h, h->q.p, nq, k, method, d_mask, h->ids.p, h->dists.p, h->stream);
+  std::memcpy(w.hq.p, queries, nq * h->dim * sizeof(float));
+  VSG_HIP(hipMemcpyAsync(w.q.p, w.hq.p, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  rc = enqueue_search(h, w, w.q.p, nq, k, method, d_mask, w.ids.p, w.dists.p, h->stream);
   if (rc) return rc;
-  VSG_HIP(hipMemcpyAsync(h->hids.p, h->ids.p, nq * k * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-  VSG_HIP(hipMemcpyAsync(h->hdists.p, h->dists.p, nq * k * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  VSG_HIP(hipMemcpyAsync(w.hids.p, w.ids.p, nq * k * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+  VSG_HIP(hipMemcpyAsync(w.hdists.p, w.dists.p, nq * k * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   VSG_HIP(hipStreamSynchronize(h->stream));
-  std::memcpy(out_ids, h->hids.p, nq * k * sizeof(uint32_t));
-  std::memcpy(out_dists, h->hdists.p, nq * k * sizeof(float));
+  std::memcpy(out_ids, w.hids.p, nq * k * sizeof(uint32_t));
+  std::memcpy(out_dists, w.hdists.p, nq * k * sizeof(float));
   return VSG_OK;
 }
 
@@ -745,54 +894,14 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
   if (nq == 0 || k == 0) return VSG_OK;
   if (!d_queries || !d_ids || !d_dists) return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
   std::lock_guard<std::mutex> g(h->mu);
+  std::shared_lock<std::shared_mutex> rl(h->rw);
   VSG_HIP(hipSetDevice(h->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : h->stream;
   if (h->n == 0) {
     VSG_HIP(vsg::launch_fill_missing(st, d_ids, d_dists, nq * k));
     return VSG_OK;
   }
-  return enqueue_search(h, d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
-}
-
-// ---- tile autotuner of the filtered scan ---------------------------------------------------
-// A call's shape class is (dim, k > 64, log2 queries per distinct range, log2 rows per distinct
-// range). The first call of a class that is large enough to time (>= 2^24 query-row pairs) runs
-// once per candidate tile, each timed on the scan kernel alone, and the class keeps the fastest.
-// Every candidate returns the same bits, so the outputs of that call are valid whichever ran
-// last.
-
-// Scan-kernel stopwatch for the tuner (search_lists_host synchronises before returning).
-struct ScanStopwatch : vsg::TimingSink {
-  hipEvent_t pending = nullptr, a = nullptr, b = nullptr;
-  ~ScanStopwatch() override {
-    for (hipEvent_t e : {pending, a, b})
-      if (e) (void)hipEventDestroy(e);
-  }
-  void begin(hipStream_t st) override {
-    if (!pending && hipEventCreate(&pending) != hipSuccess) pending = nullptr;
-    if (pending) (void)hipEventRecord(pending, st);
-  }
-  void end(hipStream_t st, int kind, uint32_t) override {
-    if (kind != VSG_KERNEL_LISTS || !pending) return;
-    if (!b && hipEventCreate(&b) != hipSuccess) b = nullptr;
-    if (!b) return;
-    std::swap(a, pending);
-    (void)hipEventRecord(b, st);
-  }
-  float ms() {
-    float v = -1.f;
-    if (a && b && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&v, a, b) == hipSuccess) return v;
-    return -1.f;
-  }
-};
-
-static int ilog2_floor(uint64_t v) {
-  int r = 0;
-  while (v > 1) {
-    v >>= 1;
-    ++r;
-  }
-  return r;
+  return enqueue_search(h, h->ws, d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
 int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
@@ -805,61 +914,9 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
   if (!queries || !ks || !range || !out_ids || !out_dists)
     return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
   std::lock_guard<std::mutex> g(h->mu);
+  std::shared_lock<std::shared_mutex> rl(h->rw);
   VSG_HIP(hipSetDevice(h->device));
-  IndexTiming tm(h);
-  const vsg::ListsTiles& fixed = h->lists_tiles;
-  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0;
-  if (planner && h->lists_autotune) {
-    std::set<std::pair<uint64_t, uint64_t>> distinct;
-    uint64_t pairs = 0, rows = 0;
-    for (size_t i = 0; i < nq; ++i) {
-      const bool all = range[2 * i] == VSG_ALL_ROWS;
-      const uint64_t len = all ? h->n : (range[2 * i + 1] >= range[2 * i] ? range[2 * i + 1] - range[2 * i] : 0);
-      pairs += len;
-      const std::pair<uint64_t, uint64_t> rk(all ? uint64_t(VSG_ALL_ROWS) : range[2 * i],
-                                             all ? uint64_t(0) : range[2 * i + 1]);
-      if (distinct.insert(rk).second)
-        rows += len;
-    }
-    const size_t nd = std::max<size_t>(1, distinct.size());
-    const auto key = std::make_tuple(h->dim, kmax > 64 ? 1 : 0, ilog2_floor(nq / nd + 1),
-                                     ilog2_floor(rows / nd + 1));
-    auto it = h->lists_tuned.find(key);
-    if (it != h->lists_tuned.end()) {
-      return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
-                                    h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids,
-                                    out_ids, out_dists, g_err, &tm, it->second, &h->lists_last);
-    }
-    if (pairs >= (uint64_t(1) << 24)) {
-      static const int cand[][3] = {{8, 2, 0},    {4, 4, 0},    {4, 4, 1024}, {4, 4, 4096},
-                                    {8, 2, 1024}, {2, 4, 0},    {4, 2, 1024}, {8, 4, 4096}};
-      vsg::ListsTiles best;
-      float best_ms = -1.f;
-      for (const auto& c : cand) {
-        vsg::ListsTiles t;
-        t.qb = c[0];
-        t.r = c[1];
-        t.chunk = uint64_t(c[2]);
-        ScanStopwatch sw;
-        vsg::ListsTiles used;
-        const int rc = vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric,
-                                              h->num_cus, h->stream, queries, nq, ks, kmax, range,
-                                              allow_ids, n_ids, out_ids, out_dists, g_err, &sw, t, &used);
-        if (rc) return rc;
-        const float ms = sw.ms();
-        if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
-          best_ms = ms;
-          best = t;
-          h->lists_last = used;
-        }
-      }
-      if (best_ms >= 0.f) h->lists_tuned[key] = best;
-      return VSG_OK;
-    }
-  }
-  return vsg::search_lists_host(h->lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
-                                h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids,
-                                out_ids, out_dists, g_err, &tm, fixed, &h->lists_last);
+  return run_lists(h, h->ws, h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists);
 }
 
 int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
@@ -873,10 +930,12 @@ int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
   else if (key == "lists_chunk") *value = int64_t(h->lists_tiles.chunk);
   else if (key == "lists_blocks_per_cu") *value = h->lists_tiles.blocks_per_cu;
   else if (key == "lists_autotune") *value = h->lists_autotune ? 1 : 0;
-  else if (key == "lists_tuned_classes") *value = int64_t(h->lists_tuned.size());
-  else if (key == "lists_last_qb") *value = h->lists_last.qb;
-  else if (key == "lists_last_r") *value = h->lists_last.r;
-  else if (key == "lists_last_chunk") *value = int64_t(h->lists_last.chunk);
+  else if (key == "lists_tuned_classes") {
+    std::lock_guard<std::mutex> t(h->tune_mu);
+    *value = int64_t(h->lists_tuned.size());
+  } else if (key == "lists_last_qb") *value = h->ws.lists_last.qb;
+  else if (key == "lists_last_r") *value = h->ws.lists_last.r;
+  else if (key == "lists_last_chunk") *value = int64_t(h->ws.lists_last.chunk);
   else return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
   return VSG_OK;
 }
@@ -884,6 +943,7 @@ int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
 int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
   if (!h || !name) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(h->mu);
+  std::unique_lock<std::shared_mutex> wl(h->rw);
   const std::string key(name);
   if (key == "scan_waves_per_cu") {
     if (value < 1 || value > 32) return fail(VSG_ERR_INVALID_ARGUMENT, "scan_waves_per_cu must be in 1..32");
@@ -916,7 +976,10 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
   if (key == "lists_autotune") {
     if (value != 0 && value != 1) return fail(VSG_ERR_INVALID_ARGUMENT, "lists_autotune must be 0 or 1");
     h->lists_autotune = value != 0;
-    if (!h->lists_autotune) h->lists_tuned.clear();
+    if (!h->lists_autotune) {
+      std::lock_guard<std::mutex> t(h->tune_mu);
+      h->lists_tuned.clear();
+    }
     return VSG_OK;
   }
   if (key == "lists_blocks_per_cu") {
@@ -941,7 +1004,12 @@ int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
   double ms[VSG_KERNEL_KINDS] = {};
   uint64_t cnt[VSG_KERNEL_KINDS] = {};
   hipError_t err = hipSuccess;
-  for (auto& t : h->timed) {
+  std::vector<vsg_index::Timed> timed;
+  {
+    std::lock_guard<std::mutex> t(h->timing_mu);
+    timed.swap(h->timed);
+  }
+  for (auto& t : timed) {
     float v = 0;
     hipError_t e = hipEventSynchronize(t.b);
     if (e == hipSuccess) e = hipEventElapsedTime(&v, t.a, t.b);
@@ -951,7 +1019,6 @@ int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
   }
-  h->timed.clear();
   if (err != hipSuccess) return fail_hip("vsg_kernel_times", err);
   for (int i = 0; i < VSG_KERNEL_KINDS; ++i) {
     if (ms_out) ms_out[i] = ms[i];
@@ -963,9 +1030,9 @@ int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
 int vsg_short_stats(vsg_index* h, uint64_t* emitted, uint64_t* rescored, uint64_t* fallbacks) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   std::lock_guard<std::mutex> g(h->mu);
-  if (emitted) *emitted = h->st_emitted;
-  if (rescored) *rescored = h->st_rescored;
-  if (fallbacks) *fallbacks = h->st_fallback;
+  if (emitted) *emitted = h->ws.st_emitted;
+  if (rescored) *rescored = h->ws.st_rescored;
+  if (fallbacks) *fallbacks = h->ws.st_fallback;
   return VSG_OK;
 }
 
@@ -981,6 +1048,332 @@ int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim) 
   if (d_base) *d_base = h->base;
   if (padded_dim) *padded_dim = h->dp;
   return VSG_OK;
+}
+
+}  // extern "C"
+
+// ---- serving loop ------------------------------------------------------------------------
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct ServeRequest {
+  const float* q = nullptr;
+  size_t nq = 0, k = 0;
+  const uint32_t* allow = nullptr;
+  size_t n_allow = 0;
+  bool filtered = false;
+  uint32_t* out_ids = nullptr;
+  float* out_dists = nullptr;
+  size_t bytes = 0;  // admission estimate
+  Clock::time_point t_submit, t_done;
+  bool done = false;
+  int status = VSG_OK;
+  std::string err;
+};
+
+struct ServeSlot {
+  hipStream_t st = nullptr;
+  Workspace w;
+  std::thread th;
+};
+
+}  // namespace
+
+struct vsg_server {
+  vsg_index* h = nullptr;
+  vsg_server_config cfg{};
+  std::mutex m;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<ServeRequest>> queue;
+  std::unordered_map<uint64_t, std::shared_ptr<ServeRequest>> live;
+  uint64_t next_ticket = 1;
+  size_t inflight = 0;  // admitted estimate in flight
+  int busy = 0;         // slots running a batch
+  bool stop = false;
+  vsg_server_stats stats{};
+  std::vector<std::unique_ptr<ServeSlot>> slots;
+};
+
+namespace {
+
+// Workspace estimate of one request: staging and device copies of its queries and results,
+// the per-query scratch of the path it takes (shortlist candidate buffers and tile minima, or
+// the filtered scan's per-chunk lists), and its allowed ids.
+size_t request_bytes(const vsg_index* h, size_t nq, size_t k, bool filtered, size_t n_allow) {
+  const size_t io = nq * (h->dim * 4 + k * 8) * 2;
+  if (!filtered) {
+    const size_t tiles = h->n / 256 + 1;
+    return io + nq * (size_t(vsg::short_cand_cap()) * 8 + tiles * 4 + 64 * k * 8);
+  }
+  const size_t chunks = n_allow / 1024 + 1;
+  return io + n_allow * 4 + nq * chunks * k * 8;
+}
+
+// One device batch on a slot. Returns the status shared by the batch's requests.
+int run_batch(vsg_server* s, ServeSlot& sl, const std::vector<std::shared_ptr<ServeRequest>>& batch) {
+  vsg_index* h = s->h;
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  VSG_HIP(hipSetDevice(h->device));
+  Workspace& w = sl.w;
+  size_t nu = 0, ku = 0;
+  for (const auto& r : batch)
+    if (!r->filtered) {
+      nu += r->nq;
+      ku = std::max(ku, r->k);
+    }
+  // a small unfiltered group joins the filtered scan's launch; a large one takes `method`
+  const bool unfiltered_in_lists = nu > 0 && nu < size_t(h->auto_short_batch);
+  if (h->n == 0) {
+    for (const auto& r : batch) fill_missing(r->out_ids, r->out_dists, r->nq * r->k);
+    return VSG_OK;
+  }
+  if (nu > 0 && !unfiltered_in_lists) {
+    hipError_t e;
+    if ((e = w.q.reserve(nu * h->dim)) != hipSuccess) return fail_hip("workspace", e);
+    if ((e = w.ids.reserve(nu * ku)) != hipSuccess) return fail_hip("workspace", e);
+    if ((e = w.dists.reserve(nu * ku)) != hipSuccess) return fail_hip("workspace", e);
+    if ((e = w.hq.reserve(nu * h->dim)) != hipSuccess) return fail_hip("pinned staging", e);
+    if ((e = w.hids.reserve(nu * ku)) != hipSuccess) return fail_hip("pinned staging", e);
+    if ((e = w.hdists.reserve(nu * ku)) != hipSuccess) return fail_hip("pinned staging", e);
+    size_t at = 0;
+    for (const auto& r : batch)
+      if (!r->filtered) {
+        std::memcpy(w.hq.p + at * h->dim, r->q, r->nq * h->dim * sizeof(float));
+        at += r->nq;
+      }
+    VSG_HIP(hipMemcpyAsync(w.q.p, w.hq.p, nu * h->dim * sizeof(float), hipMemcpyHostToDevice, sl.st));
+    int rc = enqueue_search(h, w, w.q.p, nu, ku, s->cfg.method, nullptr, w.ids.p, w.dists.p, sl.st);
+    if (rc) return rc;
+    VSG_HIP(hipMemcpyAsync(w.hids.p, w.ids.p, nu * ku * sizeof(uint32_t), hipMemcpyDeviceToHost, sl.st));
+    VSG_HIP(hipMemcpyAsync(w.hdists.p, w.dists.p, nu * ku * sizeof(float), hipMemcpyDeviceToHost, sl.st));
+    VSG_HIP(hipStreamSynchronize(sl.st));
+    at = 0;
+    for (const auto& r : batch)
+      if (!r->filtered) {
+        for (size_t i = 0; i < r->nq; ++i) {  // top-k is the first k of the top-ku
+          std::memcpy(r->out_ids + i * r->k, w.hids.p + (at + i) * ku, r->k * sizeof(uint32_t));
+          std::memcpy(r->out_dists + i * r->k, w.hdists.p + (at + i) * ku, r->k * sizeof(float));
+        }
+        at += r->nq;
+      }
+  }
+  // filtered requests (and a small unfiltered group): one persistent launch
+  size_t nl = 0, kl = 0, nid = 0;
+  for (const auto& r : batch)
+    if (r->filtered || unfiltered_in_lists) {
+      nl += r->nq;
+      kl = std::max(kl, r->k);
+      if (r->filtered) nid += r->n_allow;
+    }
+  if (nl == 0) return VSG_OK;
+  std::vector<float> q(nl * h->dim);
+  std::vector<uint32_t> ks(nl), ids(std::max<size_t>(1, nid));
+  std::vector<uint64_t> range(2 * nl);
+  size_t at = 0, idat = 0;
+  for (const auto& r : batch) {
+    if (!(r->filtered || unfiltered_in_lists)) continue;
+    std::memcpy(q.data() + at * h->dim, r->q, r->nq * h->dim * sizeof(float));
+    if (r->filtered && r->n_allow) std::memcpy(ids.data() + idat, r->allow, r->n_allow * sizeof(uint32_t));
+    for (size_t i = 0; i < r->nq; ++i) {
+      ks[at + i] = uint32_t(r->k);
+      range[2 * (at + i)] = r->filtered ? uint64_t(idat) : uint64_t(VSG_ALL_ROWS);
+      range[2 * (at + i) + 1] = r->filtered ? uint64_t(idat + r->n_allow) : 0;
+    }
+    if (r->filtered) idat += r->n_allow;
+    at += r->nq;
+  }
+  std::vector<uint32_t> oid(nl * kl);
+  std::vector<float> odist(nl * kl);
+  int rc = run_lists(h, w, sl.st, q.data(), nl, ks.data(), kl, range.data(), ids.data(), nid,
+                     oid.data(), odist.data());
+  if (rc) return rc;
+  at = 0;
+  for (const auto& r : batch) {
+    if (!(r->filtered || unfiltered_in_lists)) continue;
+    for (size_t i = 0; i < r->nq; ++i) {
+      std::memcpy(r->out_ids + i * r->k, oid.data() + (at + i) * kl, r->k * sizeof(uint32_t));
+      std::memcpy(r->out_dists + i * r->k, odist.data() + (at + i) * kl, r->k * sizeof(float));
+    }
+    at += r->nq;
+  }
+  return VSG_OK;
+}
+
+void serve_worker(vsg_server* s, ServeSlot* sl) {
+  for (;;) {
+    std::vector<std::shared_ptr<ServeRequest>> batch;
+    size_t bytes = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->m);
+      bool counted = false;
+      for (;;) {
+        if (!s->queue.empty()) {
+          const size_t head = s->queue.front()->bytes;
+          if (s->inflight == 0 || s->inflight + head <= s->cfg.workspace_bytes) break;
+          if (!counted) {
+            s->stats.admission_waits++;
+            counted = true;
+          }
+        } else if (s->stop) {
+          return;
+        }
+        s->cv_work.wait(lk);
+      }
+      size_t nq = 0;
+      while (!s->queue.empty()) {
+        const auto& r = s->queue.front();
+        if (!batch.empty() && (nq + r->nq > s->cfg.max_batch_queries ||
+                               s->inflight + bytes + r->bytes > s->cfg.workspace_bytes))
+          break;
+        nq += r->nq;
+        bytes += r->bytes;
+        batch.push_back(r);
+        s->queue.pop_front();
+      }
+      s->inflight += bytes;
+      s->busy++;
+      s->stats.peak_inflight_bytes = std::max<uint64_t>(s->stats.peak_inflight_bytes, s->inflight);
+      s->stats.max_batch_requests = std::max<uint64_t>(s->stats.max_batch_requests, batch.size());
+    }
+    const int rc = run_batch(s, *sl, batch);
+    const std::string err = rc ? g_err : std::string();
+    const auto now = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(s->m);
+      for (const auto& r : batch) {
+        r->status = rc;
+        r->err = err;
+        r->t_done = now;
+        r->done = true;
+        s->stats.requests++;
+        s->stats.queries += r->nq;
+      }
+      s->stats.batches++;
+      s->inflight -= bytes;
+      s->busy--;
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vsg_server_create(vsg_index* h, const vsg_server_config* cfg, vsg_server** out) {
+  if (!h || !out) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  vsg_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.max_batch_queries == 0) c.max_batch_queries = 4096;
+  if (c.workspace_bytes == 0) c.workspace_bytes = size_t(4) << 30;
+  if (c.slots == 0) c.slots = 2;
+  if (c.slots < 1 || c.slots > 8) return fail(VSG_ERR_INVALID_ARGUMENT, "slots must be in 1..8");
+  if (c.method < VSG_METHOD_AUTO || c.method > VSG_METHOD_FP8)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "unknown search method");
+  VSG_HIP(hipSetDevice(h->device));
+  vsg_server* s = new (std::nothrow) vsg_server;
+  if (!s) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->h = h;
+  s->cfg = c;
+  for (int i = 0; i < c.slots; ++i) {
+    auto sl = std::make_unique<ServeSlot>();
+    hipError_t e = hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      vsg_server_destroy(s);
+      return fail_hip("vsg_server_create", e);
+    }
+    s->slots.push_back(std::move(sl));
+  }
+  for (auto& sl : s->slots) sl->th = std::thread(serve_worker, s, sl.get());
+  *out = s;
+  return VSG_OK;
+}
+
+int vsg_server_submit(vsg_server* s, const float* queries, size_t nq, size_t k,
+                      const uint32_t* allow_ids, size_t n_allowed, uint32_t* out_ids,
+                      float* out_dists, uint64_t* ticket) {
+  if (!s || !ticket) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  if (k > VSG_MAX_K) return fail(VSG_ERR_INVALID_ARGUMENT, "k must be <= " + std::to_string(VSG_MAX_K));
+  if ((nq && !queries) || (nq * k && (!out_ids || !out_dists)))
+    return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
+  if (allow_ids == nullptr && n_allowed != 0) return fail(VSG_ERR_INVALID_ARGUMENT, "null allow_ids");
+  vsg_index* h = s->h;
+  {
+    std::shared_lock<std::shared_mutex> rl(h->rw);
+    for (size_t i = 0; i < n_allowed; ++i)
+      if (allow_ids[i] >= h->n || (i && allow_ids[i] <= allow_ids[i - 1]))
+        return fail(VSG_ERR_INVALID_ARGUMENT, "allowed ids must be strictly increasing and < size()");
+  }
+  auto r = std::make_shared<ServeRequest>();
+  r->q = queries;
+  r->nq = nq;
+  r->k = k;
+  r->allow = allow_ids;
+  r->n_allow = n_allowed;
+  r->filtered = allow_ids != nullptr;
+  r->out_ids = out_ids;
+  r->out_dists = out_dists;
+  r->bytes = request_bytes(h, nq, k, r->filtered, n_allowed);
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> lk(s->m);
+  if (s->stop) return fail(VSG_ERR_INVALID_ARGUMENT, "server is shutting down");
+  *ticket = s->next_ticket++;
+  s->live[*ticket] = r;
+  if (nq == 0 || k == 0) {
+    r->done = true;
+    r->t_done = r->t_submit;
+    s->stats.requests++;
+    return VSG_OK;
+  }
+  s->queue.push_back(r);
+  s->cv_work.notify_one();
+  return VSG_OK;
+}
+
+int vsg_server_wait(vsg_server* s, uint64_t ticket, double* latency_ms) {
+  if (!s) return fail(VSG_ERR_INVALID_ARGUMENT, "null server");
+  std::shared_ptr<ServeRequest> r;
+  {
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return fail(VSG_ERR_INVALID_ARGUMENT, "unknown or already waited ticket");
+    r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->live.erase(it);
+  }
+  if (latency_ms)
+    *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  if (r->status) return fail(r->status, r->err);
+  return VSG_OK;
+}
+
+int vsg_server_get_stats(vsg_server* s, vsg_server_stats* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(s->m);
+  *out = s->stats;
+  return VSG_OK;
+}
+
+void vsg_server_destroy(vsg_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& sl : s->slots)
+    if (sl->th.joinable()) sl->th.join();
+  (void)hipSetDevice(s->h->device);
+  for (auto& sl : s->slots) {
+    if (sl->st) (void)hipStreamSynchronize(sl->st);
+    sl->w.release();
+    if (sl->st) (void)hipStreamDestroy(sl->st);
+  }
+  delete s;
 }
 
 }  // extern "C"
