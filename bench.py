@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--filtered", type=int, default=4096,
                     help="queries of the batched filtered scan measurement (0 to skip)")
     ap.add_argument("--filtered-rows", type=int, default=892_800)
+    ap.add_argument("--serve-rows", type=int, default=2_000_000,
+                    help="rows of the serving-loop base (0 to skip the serving measurement)")
+    ap.add_argument("--serve-dim", type=int, default=768)
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="offered-load time per rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB megabytes (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings), then exit")
@@ -106,6 +110,109 @@ def cpu_baseline(base, queries, k, seconds, threads):
     return {"value": round(qps, 2), "unit": "queries/s", "cores": threads, "kind": kind,
             "sample": f"{calls} calls x {nq} queries against all {base.shape[0]:,} x {base.shape[1]} "
                       f"rows, k={k}, {el:.1f} s ({'BruteForceIndex::search_batch from the reference sources' if ref else 'C oracle'})"}
+
+
+def serving(args, log):
+    """Poisson arrivals of requests carrying 1-512 queries (log-uniform), k in {10, 100}, half
+    with a Zipf group filter over 100 groups, against a (rows x dim) base held with its BF16
+    copy in HBM. The saturation rate is measured first (every request of a burst submitted at
+    once); the five offered rates are fractions of it. Latency is submit -> completion as the
+    server measures it."""
+    import threading
+
+    import vecsearch_amd as vs
+    from vecsearch_amd import datagen
+
+    n, d = args.serve_rows, args.serve_dim
+    t = time.time()
+    x = datagen.mixture(n, d, seed=10, workers=16)
+    groups = datagen.zipf_groups(n, 100, 1.1, seed=11)
+    goff, gids = datagen.group_csr(groups, 100)
+    gids = gids.astype(np.uint32)
+    log(f"serving: {n:,} x {d} base generated in {time.time() - t:.1f} s")
+    idx = vs.GpuBruteForceIndex(d, "cosine")
+    t = time.time()
+    idx.add(x)
+    del x
+    log(f"  uploaded in {time.time() - t:.1f} s")
+    rng = np.random.default_rng(12)
+    pool_q = datagen.mixture(4096, d, seed=13)
+
+    def make_request():
+        nq = int(np.floor(np.exp(rng.uniform(0.0, np.log(513.0)))))
+        nq = max(1, min(512, nq))
+        off = int(rng.integers(0, len(pool_q) - nq + 1))
+        k = int(rng.choice([10, 100]))
+        allow = None
+        if rng.random() < 0.5:
+            g = int(rng.integers(0, 100))
+            allow = np.sort(gids[goff[g]:goff[g + 1]])
+        return pool_q[off:off + nq], k, allow
+
+    srv = vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=2, method="auto")
+    # warm-up: builds the BF16 copy and tunes the filtered-scan tiles for the common shapes
+    warm = [make_request() for _ in range(16)]
+    for r in warm:
+        srv.wait(srv.submit(*r))
+
+    def run(reqs, rate):
+        """Submit reqs at Poisson times (rate requests/s; 0 = all at once); returns latencies."""
+        tickets, lat = [], []
+        done = threading.Event()
+
+        def waiter():
+            i = 0
+            while i < len(reqs):
+                while i >= len(tickets):
+                    time.sleep(0.0002)
+                _, _, ms = srv.wait(tickets[i])
+                lat.append(ms)
+                i += 1
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        next_t = t0
+        for r in reqs:
+            if rate > 0:
+                next_t += rng.exponential(1.0 / rate)
+                dt = next_t - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            tickets.append(srv.submit(*r))
+        done.wait()
+        th.join()
+        return np.array(lat), time.perf_counter() - t0
+
+    burst = [make_request() for _ in range(400)]
+    lat, el = run(burst, 0)
+    sat_rps = len(burst) / el
+    sat_qps = sum(len(r[0]) for r in burst) / el
+    log(f"  saturation: {sat_rps:,.0f} requests/s, {sat_qps:,.0f} queries/s")
+    rows = []
+    for frac in (0.1, 0.25, 0.5, 0.75, 0.9):
+        rate = sat_rps * frac
+        nreq = int(max(50, min(4000, rate * args.serve_seconds)))
+        reqs = [make_request() for _ in range(nreq)]
+        st0 = srv.stats()
+        lat, el = run(reqs, rate)
+        st1 = srv.stats()
+        nqs = sum(len(r[0]) for r in reqs)
+        rows.append({"offered_requests_per_s": round(rate, 1), "fraction_of_saturation": frac,
+                     "requests": nreq, "qps": round(nqs / el, 1),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "batches": st1["batches"] - st0["batches"]})
+        log(f"  rate {rate:,.0f}/s: {rows[-1]}")
+    st = srv.stats()
+    srv.close()
+    return {"workload": f"{n:,} x {d} fp32 cosine base + BF16 copy resident; requests of 1-512 queries "
+                        f"(log-uniform), k in {{10, 100}}, 50% with one Zipf(s=1.1) group filter of 100",
+            "server": {"slots": 2, "max_batch_queries": 4096, "workspace_bytes": 8 << 30},
+            "saturation_requests_per_s": round(sat_rps, 1), "saturation_qps": round(sat_qps, 1),
+            "rates": rows, "admission_waits": st["admission_waits"],
+            "peak_inflight_bytes": st["peak_inflight_bytes"], "max_batch_requests": st["max_batch_requests"]}
 
 
 def main():
@@ -353,6 +460,12 @@ def main():
         sweep[str(b)] = row
         log(f"  sweep batch {b}: {row}")
 
+    serve = {}
+    if args.serve_rows > 0:
+        del idx  # free the 1M base before the serving base is uploaded
+        torch.cuda.synchronize()
+        serve = serving(args, log)
+
     cpu = None
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -380,6 +493,7 @@ def main():
         "cpu_baseline": cpu,
         "shortlist": shortlist,
         "filtered": filtered,
+        "serving": serve,
         "sweep": sweep,
     }
     print(json.dumps(out), flush=True)

@@ -26,15 +26,32 @@ def centers(d, centers_seed=0, n_clusters=N_CLUSTERS):
     return np.random.default_rng([centers_seed, 7]).standard_normal((n_clusters, d), dtype=np.float32)
 
 
-def mixture(n, d, seed, centers_seed=0, n_clusters=N_CLUSTERS, sigma=SIGMA, slice_rows=100_000):
+def mixture(n, d, seed, centers_seed=0, n_clusters=N_CLUSTERS, sigma=SIGMA, slice_rows=100_000,
+            workers=1):
+    """workers > 1 draws the slices in parallel threads, slice i from its own generator
+    seeded (seed, 11, i): a different, equally deterministic stream than workers = 1."""
     c = centers(d, centers_seed, n_clusters)
-    rng = np.random.default_rng([seed, 11])
     x = np.empty((n, d), np.float32)
-    for i in range(0, n, slice_rows):
+    if workers <= 1:
+        rng = np.random.default_rng([seed, 11])
+        for i in range(0, n, slice_rows):
+            m = min(slice_rows, n - i)
+            lab = rng.integers(0, n_clusters, m)
+            x[i:i + m] = c[lab] + np.float32(sigma) * rng.standard_normal((m, d), dtype=np.float32)
+        return _normalize_(x)
+    from concurrent.futures import ThreadPoolExecutor
+
+    def fill(j):
+        i = j * slice_rows
         m = min(slice_rows, n - i)
+        rng = np.random.default_rng([seed, 11, j])
         lab = rng.integers(0, n_clusters, m)
         x[i:i + m] = c[lab] + np.float32(sigma) * rng.standard_normal((m, d), dtype=np.float32)
-    return _normalize_(x)
+        _normalize_(x[i:i + m])
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(fill, range((n + slice_rows - 1) // slice_rows)))
+    return x
 
 
 def unit_vectors(n, d, seed, slice_rows=100_000):
