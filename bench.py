@@ -154,6 +154,9 @@ def serving(args, log):
     warm = [make_request() for _ in range(16)]
     for r in warm:
         srv.wait(srv.submit(*r))
+    warm = [make_request() for _ in range(64)]
+    for t in [srv.submit(*r) for r in warm]:
+        srv.wait(t)
 
     def run(reqs, rate):
         """Submit reqs at Poisson times (rate requests/s; 0 = all at once); returns latencies."""
@@ -212,7 +215,8 @@ def serving(args, log):
             "server": {"slots": 2, "max_batch_queries": 4096, "workspace_bytes": 8 << 30},
             "saturation_requests_per_s": round(sat_rps, 1), "saturation_qps": round(sat_qps, 1),
             "rates": rows, "admission_waits": st["admission_waits"],
-            "peak_inflight_bytes": st["peak_inflight_bytes"], "max_batch_requests": st["max_batch_requests"]}
+            "peak_inflight_bytes": st["peak_inflight_bytes"], "max_batch_requests": st["max_batch_requests"],
+            "shortlist_fallbacks": st["shortlist_fallbacks"]}
 
 
 def main():
@@ -443,7 +447,7 @@ def main():
             continue
         row = {}
         row["qps_plain"] = round(device_qps(b, "skinny", args.sweep_seconds)[0], 1)
-        if b >= 32:
+        if b >= 8:
             for m in ("bf16", "fp8"):
                 row[f"qps_{m}"] = round(device_qps(b, m, args.sweep_seconds)[0], 1)
         lat = []

@@ -45,7 +45,7 @@ typedef struct vsg_index vsg_index;
  *   BF16    the same as TILED;
  *   FP8     the certified shortlist with an FP8 E4M3 (per-row scale) MFMA scan;
  *   AUTO    SKINNY for small batches, TILED for large ones.
- * The shortlist applies to inner product / cosine with dim <= 384 on bases of at least
+ * The shortlist applies to inner product / cosine with dim <= 768 on bases of at least
  * 65,536 rows; elsewhere every method runs the exact scan. */
 #define VSG_METHOD_AUTO 0
 #define VSG_METHOD_NAIVE 1
@@ -158,9 +158,12 @@ int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim);
  * arrive meanwhile join the next free slot's batch, so batches overlap on the device. A request
  * is admitted only while the estimated workspace bytes of the requests in flight stay within
  * `workspace_bytes` (the first request of an idle server is always admitted). Inside a batch,
- * filtered requests (and unfiltered ones when fewer than the index's auto_short_batch) run in
- * one persistent launch of the filtered scan; larger unfiltered groups take `method`. Results
+ * filtered requests run in one persistent launch of the filtered scan and the unfiltered
+ * queries, gathered into one block, take `method` (AUTO: the streaming scan for small blocks,
+ * the certified shortlist for large ones). Results
  * equal the index's own search of the same request (bit-identical), alone or in any mix.
+ * With method VSG_METHOD_NAIVE every request of a batch, filtered or not, runs in the one
+ * persistent launch of the filtered scan (unfiltered ones as VSG_ALL_ROWS ranges).
  * The caller keeps queries, allow_ids and the output buffers alive until the request is
  * waited for. While a server exists, vsg_add / vsg_reserve wait for in-flight batches. */
 typedef struct vsg_server vsg_server;
@@ -176,6 +179,8 @@ typedef struct {
   uint64_t admission_waits;            /* times a free slot waited for workspace budget */
   uint64_t peak_inflight_bytes;        /* largest admitted estimate in flight */
   uint64_t max_batch_requests;         /* most requests in one batch */
+  uint64_t shortlist_fallbacks;        /* unfiltered queries answered by the exact scan after
+                                          their shortlist candidates overflowed */
 } vsg_server_stats;
 
 int vsg_server_create(vsg_index* h, const vsg_server_config* cfg, vsg_server** out);

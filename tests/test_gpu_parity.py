@@ -281,18 +281,23 @@ def test_shortlist_unnormalized_ip(scale):
         assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
 
 
-def test_shortlist_overflow_falls_back_exactly():
-    # 3,000 identical rows tie with each other: the candidate set exceeds the buffer and the
-    # queries near them must be answered by the exact scan, with ties to the lower id.
+@pytest.mark.parametrize("dups", [3000, 9000])
+def test_shortlist_many_ties_and_overflow(dups):
+    # `dups` identical rows tie with each other. 3,000 fit the candidate buffer (8,192) and are
+    # rescored and ranked in LDS; 9,000 overflow it, and the queries near them are answered by
+    # the exact scan. Either way ties go to the lower id.
     x = dg.mixture(70_000, 128, seed=44)
-    x[10_000:13_000] = x[5]
-    q = np.concatenate([x[5:6] * 0.999 + 0.001, dg.mixture(63, 128, seed=45)]).astype(np.float32)
+    x[10_000:10_000 + dups] = x[5]
+    near = x[5:6] * 0.999 + 0.001 * dg.mixture(4, 128, seed=46)
+    q = np.concatenate([near[:1], dg.mixture(40, 128, seed=45), near[1:],
+                        dg.mixture(20, 128, seed=47)]).astype(np.float32)
     q = vs.normalize(q)
     gpu = vs.GpuBruteForceIndex(128, "cosine")
     gpu.add(x)
     got = gpu.search(q, 50, method="bf16")
     assert_same(got, oracle.search(x, q, 50, "ip", threads=8))
-    assert gpu.short_stats()["fallbacks"] >= 1
+    fallbacks = gpu.short_stats()["fallbacks"]
+    assert fallbacks >= 4 if dups > 8192 else fallbacks == 0
     assert list(got[0][0, :50]) == [5] + list(range(10_000, 10_049))
 
 

@@ -73,6 +73,18 @@ def test_same_alone_and_in_the_mix(served):
             assert np.isin(mi[mi >= 0], allow).all()
 
 
+def test_one_persistent_launch_mode(served):
+    # method "naive": unfiltered requests join the filtered scan's launch as ALL_ROWS ranges
+    x, gpu, reqs = served
+    with vs.Server(gpu, max_batch_queries=512, slots=1, method="naive") as srv:
+        tickets = [srv.submit(q, k, allow) for q, k, allow in reqs[:10]]
+        got = [srv.wait(t) for t in tickets]
+    for (q, k, allow), (gi, gd, _) in zip(reqs[:10], got):
+        ei, ed = expected(x, q, k, allow)
+        np.testing.assert_array_equal(gi, as_i64(ei))
+        np.testing.assert_array_equal(bits(gd), bits(ed))
+
+
 def test_admission_by_workspace_budget(served):
     x, gpu, reqs = served
     # a budget below two requests' estimates: requests are admitted one batch at a time

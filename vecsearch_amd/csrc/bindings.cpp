@@ -141,6 +141,7 @@ class PyServer {
     d["admission_waits"] = st.admission_waits;
     d["peak_inflight_bytes"] = st.peak_inflight_bytes;
     d["max_batch_requests"] = st.max_batch_requests;
+    d["shortlist_fallbacks"] = st.shortlist_fallbacks;
     return d;
   }
 

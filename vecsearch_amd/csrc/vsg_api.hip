@@ -123,6 +123,10 @@ struct Workspace {
   DevBuf<uint64_t> s_emit;
   DevBuf<uint32_t> s_emit_count, s_emit_over;
   HostBuf<uint32_t> s_hover;
+  DevBuf<uint32_t> f_sel;  // overflowed queries of a round (exact fallback)
+  DevBuf<float> f_q;
+  DevBuf<uint32_t> f_ids;
+  DevBuf<float> f_dists;
   uint64_t st_emitted = 0, st_rescored = 0, st_fallback = 0;
   size_t st_nq = 0;
   // exact scan and host staging
@@ -154,6 +158,10 @@ struct Workspace {
     s_emit_count.release();
     s_emit_over.release();
     s_hover.release();
+    f_sel.release();
+    f_q.release();
+    f_ids.release();
+    f_dists.release();
     q.release();
     part.release();
     ids.release();
@@ -467,10 +475,13 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   if ((e = w.s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
-  // pass-2 emission regions, one per wave (64 rows): a round of 4,096 queries that each keep a
-  // few hundred candidates out of n rows fills a region to a few dozen entries on average;
-  // 1,024 leaves a wide margin for clustered data (an overflow sends the round to the exact scan)
-  const uint32_t wave_cap = 1024;
+  // pass-2 emission regions, one per wave: a region of R rows receives about
+  // round * (candidates per query) * R / n entries (rows are in no particular order); sized for
+  // 4x that at 2,048 candidates per query and at least 1,024, the most a region can receive
+  // being R * round (an overflow sends the round to the exact scan)
+  const size_t region_rows = tile_rows / size_t(vsg::short_waves());
+  const size_t expect = round * 2048 * region_rows / std::max<size_t>(1, h->n);
+  const uint32_t wave_cap = uint32_t(std::min(region_rows * round, std::max<size_t>(1024, 4 * expect)));
   const size_t n_regions = size_t(n_tiles) * vsg::short_waves();
   if ((e = w.s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
@@ -567,16 +578,26 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     }
     w.st_nq += rq;
     if (nover) {
-      // overflowed queries: exact scan, one query at a time into this round's output rows
+      // overflowed queries: gathered into one block, answered by the exact scan together,
+      // scattered back into this round's output rows
       flags.resize(rq);
       VSG_HIP(hipMemcpy(flags.data(), w.s_flags.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      for (uint32_t i = 0; i < rq; ++i) {
-        if (!flags[i]) continue;
-        w.st_fallback++;
-        rc = enqueue_search(h, w, qr + size_t(i) * h->dim, 1, k, VSG_METHOD_SKINNY, nullptr, ids + size_t(i) * k,
-                            dists + size_t(i) * k, st);
-        if (rc) return rc;
-      }
+      std::vector<uint32_t> sel;
+      for (uint32_t i = 0; i < rq; ++i)
+        if (flags[i]) sel.push_back(i);
+      const uint32_t nf = uint32_t(sel.size());
+      w.st_fallback += nf;
+      if ((e = w.f_sel.reserve(nf)) != hipSuccess) return fail_hip("workspace", e);
+      if ((e = w.f_q.reserve(size_t(nf) * h->dim)) != hipSuccess) return fail_hip("workspace", e);
+      if ((e = w.f_ids.reserve(size_t(nf) * k)) != hipSuccess) return fail_hip("workspace", e);
+      if ((e = w.f_dists.reserve(size_t(nf) * k)) != hipSuccess) return fail_hip("workspace", e);
+      VSG_HIP(hipMemcpy(w.f_sel.p, sel.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice));
+      VSG_HIP(vsg::launch_rows_move(st, qr, w.f_q.p, w.f_sel.p, nf, uint32_t(h->dim), false));
+      rc = enqueue_exact(h, w, w.f_q.p, nf, k, VSG_METHOD_SKINNY, nullptr, w.f_ids.p, w.f_dists.p, st);
+      if (rc) return rc;
+      VSG_HIP(vsg::launch_rows_move(st, w.f_ids.p, ids, w.f_sel.p, nf, uint32_t(k), true));
+      VSG_HIP(vsg::launch_rows_move(st, w.f_dists.p, dists, w.f_sel.p, nf, uint32_t(k), true));
+      VSG_HIP(hipStreamSynchronize(st));  // sel / f_* are reused by the next round
     }
   }
   return VSG_OK;
@@ -1123,8 +1144,10 @@ int run_batch(vsg_server* s, ServeSlot& sl, const std::vector<std::shared_ptr<Se
       nu += r->nq;
       ku = std::max(ku, r->k);
     }
-  // a small unfiltered group joins the filtered scan's launch; a large one takes `method`
-  const bool unfiltered_in_lists = nu > 0 && nu < size_t(h->auto_short_batch);
+  // Unfiltered queries take `method` (the streaming scan reads the tiled base, the shortlist
+  // the low-precision copy); an unfiltered query joins the filtered scan's launch only when
+  // the server was asked for it (method NAIVE keeps one persistent launch per batch).
+  const bool unfiltered_in_lists = nu > 0 && s->cfg.method == VSG_METHOD_NAIVE;
   if (h->n == 0) {
     for (const auto& r : batch) fill_missing(r->out_ids, r->out_dists, r->nq * r->k);
     return VSG_OK;
@@ -1237,7 +1260,9 @@ void serve_worker(vsg_server* s, ServeSlot* sl) {
       s->stats.peak_inflight_bytes = std::max<uint64_t>(s->stats.peak_inflight_bytes, s->inflight);
       s->stats.max_batch_requests = std::max<uint64_t>(s->stats.max_batch_requests, batch.size());
     }
+    sl->w.st_fallback = 0;
     const int rc = run_batch(s, *sl, batch);
+    const uint64_t fb = sl->w.st_fallback;
     const std::string err = rc ? g_err : std::string();
     const auto now = Clock::now();
     {
@@ -1251,6 +1276,7 @@ void serve_worker(vsg_server* s, ServeSlot* sl) {
         s->stats.queries += r->nq;
       }
       s->stats.batches++;
+      s->stats.shortlist_fallbacks += fb;
       s->inflight -= bytes;
       s->busy--;
     }

@@ -23,6 +23,9 @@ hipError_t launch_merge(uint32_t n_out, const uint64_t* part, const uint64_t* li
                         uint32_t L, const uint32_t* ks, uint32_t k, const uint32_t* qmap,
                         uint32_t* ids, float* dists, uint32_t ldo, hipStream_t st);
 hipError_t launch_fill_missing(hipStream_t st, uint32_t* ids, float* dists, size_t count);
+// rows of `width` 32-bit words: dst[i] = src[idx[i]] (gather) or dst[idx[i]] = src[i] (scatter)
+hipError_t launch_rows_move(hipStream_t st, const void* src, void* dst, const uint32_t* idx,
+                            uint32_t rows, uint32_t width, bool scatter);
 hipError_t launch_tile_rows(hipStream_t st, const float* base, float* tiles, uint32_t r0, uint32_t r1,
                             uint32_t dp);
 

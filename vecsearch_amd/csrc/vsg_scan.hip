@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void scan_exact_kernel(
 // 1. Threshold: m = ceil(kq / #lists) leading keys of every list form a sample of >= kq keys;
 //    its kq-th smallest key T is >= the answer's kq-th key (a subset's kq-th smallest is never
 //    below the whole set's), and because the lists are sorted and cover disjoint rows, T is
-//    close to it. The sample is ranked in LDS (by counting up to 1024 keys, else by a bitonic
+//    close to it. The sample is ranked in LDS (by counting up to 128 keys, else by a bitonic
 //    sort of up to kGather keys).
 // 2. Every thread walks its lists from the head while keys are <= T and appends them to LDS.
 // 3. The gathered keys (about kq of them) are ranked: by counting (<= 256 keys) or by a
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
       buf[i] = lists[size_t(l) * L + p];
     }
     __syncthreads();
-    if (ns <= 1024) {
+    if (ns <= 128) {  // rank by counting: ns LDS reads per key
       for (uint32_t i = tid; i < ns; i += 256) {
         const uint64_t v = buf[i];
         uint32_t below = 0, equal_before = 0;
@@ -210,8 +210,8 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
       }
       __syncthreads();
       T = red[0];
-    } else {
-      uint32_t n2 = 2048;
+    } else {  // bitonic sort: log2(n2)^2 / 2 barrier-separated stages
+      uint32_t n2 = 256;
       while (n2 < ns) n2 <<= 1;
       for (uint32_t i = ns + tid; i < n2; i += 256) buf[i] = kPadKey;
       __syncthreads();
@@ -305,6 +305,22 @@ __global__ void tile_rows_kernel(const float* __restrict__ base, float* __restri
        i += size_t(gridDim.x) * blockDim.x) {
     const uint32_t r = r0 + uint32_t(i / dp), j = uint32_t(i % dp);
     tiles[size_t(r >> 4) * 16 * dp + (j >> 4) * 256 + (r & 15) * 16 + (j & 15)] = base[size_t(r) * dp + j];
+  }
+}
+
+// Row gather / scatter of 32-bit words: dst[i] = src[idx[i]] (gather) or dst[idx[i]] = src[i]
+// (scatter), rows of `width` words, for i < rows.
+__global__ void rows_move_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                 const uint32_t* __restrict__ idx, uint32_t rows, uint32_t width,
+                                 int scatter) {
+  const size_t total = size_t(rows) * width;
+  for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+       t += size_t(gridDim.x) * blockDim.x) {
+    const uint32_t i = uint32_t(t / width), c = uint32_t(t - size_t(i) * width);
+    if (scatter)
+      dst[size_t(idx[i]) * width + c] = src[t];
+    else
+      dst[t] = src[size_t(idx[i]) * width + c];
   }
 }
 
@@ -402,6 +418,16 @@ hipError_t launch_tile_rows(hipStream_t st, const float* base, float* tiles, uin
   const size_t total = size_t(r1 - r0) * dp;
   const unsigned blocks = unsigned(std::min<size_t>(8192, (total + 255) / 256));
   hipLaunchKernelGGL(tile_rows_kernel, dim3(blocks), dim3(256), 0, st, base, tiles, r0, r1, dp);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_move(hipStream_t st, const void* src, void* dst, const uint32_t* idx,
+                            uint32_t rows, uint32_t width, bool scatter) {
+  if (rows == 0 || width == 0) return hipSuccess;
+  const size_t total = size_t(rows) * width;
+  const unsigned blocks = unsigned(std::min<size_t>((total + 255) / 256, 4096));
+  hipLaunchKernelGGL(rows_move_kernel, dim3(blocks), dim3(256), 0, st, static_cast<const uint32_t*>(src),
+                     static_cast<uint32_t*>(dst), idx, rows, width, scatter ? 1 : 0);
   return hipGetLastError();
 }
 

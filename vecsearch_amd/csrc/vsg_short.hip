@@ -526,7 +526,27 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
 }
 
 // ---- rescore -----------------------------------------------------------------------------
-constexpr int kCandCap = 2048;
+// Up to kCandCap candidates per query; a query with more is flagged and answered by the exact
+// scan. Candidates, then the exact keys, live in one dynamic LDS buffer of kCandCap keys
+// (64 KB) followed by the query (dp floats).
+constexpr int kCandCap = 8192;
+
+// Ascending bitonic sort of buf[0, n2) (n2 a power of two) by a 256-thread block.
+__device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
+  for (uint32_t size = 2; size <= n2; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = tid; i < n2 / 2; i += 256) {
+        const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = buf[lo], b = buf[hi];
+        if ((a > b) == up) {
+          buf[lo] = b;
+          buf[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+}
 
 template <int D16, bool L2>
 __global__ __launch_bounds__(256) void rescore_kernel(
@@ -536,10 +556,10 @@ __global__ __launch_bounds__(256) void rescore_kernel(
     uint32_t* __restrict__ out_ids, float* __restrict__ out_dists, uint32_t ldo,
     uint32_t* __restrict__ overflow_flags, uint32_t* __restrict__ n_overflow,
     uint32_t* __restrict__ rescored) {
-  __shared__ uint64_t cbuf[kCandCap];
-  __shared__ uint64_t ebuf[kCandCap];
+  extern __shared__ uint64_t dyn[];
+  uint64_t* buf = dyn;                                      // kCandCap keys
+  float* q_lds = reinterpret_cast<float*>(dyn + kCandCap);  // dp floats
   __shared__ uint32_t nsel;
-  extern __shared__ float q_lds[];  // dp floats
   const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
   const uint32_t kq = min(ks ? ks[q] : k, ldo);
   uint32_t* oi = out_ids + size_t(q) * ldo;
@@ -562,52 +582,53 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   // candidates sorted by (dh, id)
   uint32_t n2 = 256;
   while (n2 < c) n2 <<= 1;
-  for (uint32_t i = tid; i < n2; i += 256) cbuf[i] = i < c ? cand[size_t(q) * kCandCap + i] : kPadKey;
+  for (uint32_t i = tid; i < n2; i += 256) buf[i] = i < c ? cand[size_t(q) * kCandCap + i] : kPadKey;
   __syncthreads();
-  for (uint32_t size = 2; size <= n2; size <<= 1)
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = tid; i < n2 / 2; i += 256) {
-        const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint64_t a = cbuf[lo], b = cbuf[hi];
-        if ((a > b) == up) {
-          cbuf[lo] = b;
-          cbuf[hi] = a;
-        }
-      }
-      __syncthreads();
-    }
+  bitonic_sort_lds(buf, n2, tid);
   // keep dh <= Dk + 2E
   const uint32_t kk = min(kq, c);
   if (tid == 0) nsel = 0;
   __syncthreads();
   if (kk > 0) {
-    const float lim = round_up_f(double(key_dist(cbuf[kk - 1])) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    const float lim = round_up_f(double(key_dist(buf[kk - 1])) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
     for (uint32_t i = tid; i < c; i += 256)
-      if (key_dist(cbuf[i]) <= lim) atomicMax(&nsel, i + 1);
+      if (key_dist(buf[i]) <= lim) atomicMax(&nsel, i + 1);
   }
   __syncthreads();
   const uint32_t r = nsel;  // sorted, so the kept candidates are a prefix
   if (tid == 0) rescored[q] = r;
-  // exact distances, 16 rows per wave instruction group, 4 waves
+  // exact distances, 16 rows per wave instruction group, 4 waves; a quad reads its
+  // candidate's row id and later overwrites that slot with the exact key
   for (uint32_t base_i = (tid >> 6) * 16; base_i < r; base_i += 64) {
     const uint32_t i = base_i + g;
     bool ok[1] = {i < r};
-    const uint32_t row = key_id(cbuf[ok[0] ? i : 0]);
+    const uint32_t row = key_id(buf[ok[0] ? i : 0]);
     const float* xr[1] = {base + size_t(row) * dp};
     float dist[1][1];
     quad_dists<1, 1, D16, L2, 16, kLoadPlain>(xr, ok, q_lds, dp, d, int(t), dist);
-    if (ok[0] && t == 0) ebuf[i] = make_key(dist[0][0], row);
+    if (ok[0] && t == 0) buf[i] = make_key(dist[0][0], row);
   }
   __syncthreads();
   // rank the exact keys (unique) and write the first kq
-  for (uint32_t i = tid; i < r; i += 256) {
-    const uint64_t v = ebuf[i];
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < r; ++j) rank += ebuf[j] < v;
-    if (rank < kq) {
-      oi[rank] = key_id(v);
-      od[rank] = key_dist(v);
+  if (r <= 256) {
+    for (uint32_t i = tid; i < r; i += 256) {
+      const uint64_t v = buf[i];
+      uint32_t rank = 0;
+      for (uint32_t j = 0; j < r; ++j) rank += buf[j] < v;
+      if (rank < kq) {
+        oi[rank] = key_id(v);
+        od[rank] = key_dist(v);
+      }
+    }
+  } else {
+    uint32_t m2 = 512;
+    while (m2 < r) m2 <<= 1;
+    for (uint32_t i = r + tid; i < m2; i += 256) buf[i] = kPadKey;
+    __syncthreads();
+    bitonic_sort_lds(buf, m2, tid);
+    for (uint32_t i = tid; i < kq && i < r; i += 256) {
+      oi[i] = key_id(buf[i]);
+      od[i] = key_dist(buf[i]);
     }
   }
   for (uint32_t i = r + tid; i < kq; i += 256) {
@@ -733,16 +754,23 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
                           uint32_t* rescored) {
-  const size_t lds = size_t(dp) * sizeof(float);
-#define VSG_RESCORE(D16, L2V)                                                                        \
-  hipLaunchKernelGGL((rescore_kernel<D16, L2V>), dim3(nq), dim3(256), lds, st, base, dp, d, queries, ks, \
-                     k, E, cand, count, ids, dists, ldo, flags, n_overflow, rescored)
+  const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
+#define VSG_RESCORE(D16, L2V)                                                                         \
+  do {                                                                                                \
+    hipError_t e_ = hipFuncSetAttribute(reinterpret_cast<const void*>(&rescore_kernel<D16, L2V>),      \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));         \
+    if (e_ != hipSuccess) return e_;                                                                  \
+    hipLaunchKernelGGL((rescore_kernel<D16, L2V>), dim3(nq), dim3(256), lds, st, base, dp, d, queries,  \
+                       ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow, rescored);           \
+  } while (0)
   if (l2) {
     if (d == 384) VSG_RESCORE(24, true);
+    else if (d == 768) VSG_RESCORE(48, true);
     else if (d == 128) VSG_RESCORE(8, true);
     else VSG_RESCORE(0, true);
   } else {
     if (d == 384) VSG_RESCORE(24, false);
+    else if (d == 768) VSG_RESCORE(48, false);
     else if (d == 128) VSG_RESCORE(8, false);
     else VSG_RESCORE(0, false);
   }
