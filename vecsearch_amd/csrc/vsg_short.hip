@@ -527,9 +527,11 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
 
 // ---- rescore -----------------------------------------------------------------------------
 // Up to kCandCap candidates per query; a query with more is flagged and answered by the exact
-// scan. Candidates, then the exact keys, live in one dynamic LDS buffer of kCandCap keys
-// (64 KB) followed by the query (dp floats).
+// scan. Candidates, then the exact keys, live in one dynamic LDS buffer of CAP keys followed by
+// the query (dp floats). Two launches: CAP = kCandSmall for every query with at most that many
+// candidates (small LDS, high occupancy), then CAP = kCandCap for the rest.
 constexpr int kCandCap = 8192;
+constexpr int kCandSmall = 2048;
 
 // Ascending bitonic sort of buf[0, n2) (n2 a power of two) by a 256-thread block.
 __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
@@ -548,25 +550,27 @@ __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
     }
 }
 
-template <int D16, bool L2>
+// Handles the queries with cmin <= count <= CAP; with final_pass set, flags those above CAP.
+template <int D16, bool L2, int CAP>
 __global__ __launch_bounds__(256) void rescore_kernel(
     const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ queries,
     const uint32_t* __restrict__ ks, uint32_t k, const float* __restrict__ E,
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ count,
     uint32_t* __restrict__ out_ids, float* __restrict__ out_dists, uint32_t ldo,
     uint32_t* __restrict__ overflow_flags, uint32_t* __restrict__ n_overflow,
-    uint32_t* __restrict__ rescored) {
+    uint32_t* __restrict__ rescored, uint32_t cmin, int final_pass) {
   extern __shared__ uint64_t dyn[];
-  uint64_t* buf = dyn;                                      // kCandCap keys
-  float* q_lds = reinterpret_cast<float*>(dyn + kCandCap);  // dp floats
+  uint64_t* buf = dyn;                                // CAP keys
+  float* q_lds = reinterpret_cast<float*>(dyn + CAP);  // dp floats
   __shared__ uint32_t nsel;
   const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
   const uint32_t kq = min(ks ? ks[q] : k, ldo);
   uint32_t* oi = out_ids + size_t(q) * ldo;
   float* od = out_dists + size_t(q) * ldo;
   const uint32_t c = count[q];
-  if (c > uint32_t(kCandCap)) {
-    if (tid == 0) {
+  if (c < cmin) return;  // done by the previous launch
+  if (c > uint32_t(CAP)) {
+    if (final_pass && tid == 0) {
       overflow_flags[q] = 1;
       rescored[q] = 0;
       atomicAdd(n_overflow, 1u);
@@ -754,14 +758,21 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
                           uint32_t* rescored) {
+  // every query's flag is set or cleared by the launch that handles it
+  const size_t lds_small = size_t(kCandSmall) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
 #define VSG_RESCORE(D16, L2V)                                                                         \
   do {                                                                                                \
-    hipError_t e_ = hipFuncSetAttribute(reinterpret_cast<const void*>(&rescore_kernel<D16, L2V>),      \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));         \
+    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall>), dim3(nq), dim3(256), lds_small, st,      \
+                       base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow, \
+                       rescored, 0u, 0);                                                              \
+    hipError_t e_ = hipFuncSetAttribute(                                                              \
+        reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap>),                           \
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                        \
     if (e_ != hipSuccess) return e_;                                                                  \
-    hipLaunchKernelGGL((rescore_kernel<D16, L2V>), dim3(nq), dim3(256), lds, st, base, dp, d, queries,  \
-                       ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow, rescored);           \
+    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap>), dim3(nq), dim3(256), lds, st, base, dp,   \
+                       d, queries, ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow,          \
+                       rescored, uint32_t(kCandSmall) + 1, 1);                                        \
   } while (0)
   if (l2) {
     if (d == 384) VSG_RESCORE(24, true);
