@@ -149,6 +149,19 @@ int vsg_synchronize(vsg_index* h);
  * for measurement code that needs the exact HBM footprint. */
 int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim);
 
+/* ---- captured searches (no counterpart in the reference) ----
+ * vsg_graph_create records one exact search of nq device-resident queries (the streaming scan
+ * and its merge, method NAIVE / SKINNY / AUTO below the shortlist batch) into a HIP graph with
+ * its own workspace; vsg_graph_launch replays it on `stream`, reading whatever d_queries holds
+ * at that point and writing d_ids / d_dists. The buffers and the index must outlive the graph.
+ * The graph is bound to the base's size and address at capture: after vsg_add, a launch
+ * returns INVALID (capture a new graph). */
+typedef struct vsg_graph vsg_graph;
+int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
+                     uint32_t* d_ids, float* d_dists, vsg_graph** out);
+int vsg_graph_launch(vsg_graph* g, void* stream);
+void vsg_graph_destroy(vsg_graph* g);
+
 /* ---- serving loop (no counterpart in the reference: its GPU index serialises every call
  * behind one mutex, gpu_brute_force.cu:351,430) ----
  * Requests (a block of queries, one k, optionally an allowed-id list) are queued by

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# PyTorch-ROCm ships its own HIP runtime. Loading it before the engine lets the engine bind to
+# that same runtime (one HIP runtime per process), which the GPU tests that hand torch device
+# buffers to the engine rely on.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -8,6 +8,7 @@
 // Added: search(..., filter=None) with the HNSW binding's mask convention (:84-91), and
 // search_batch for per-query k and per-query allowed-id lists.
 #include <algorithm>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -71,6 +72,33 @@ py::array_t<int64_t> ids_to_int64(const py::array_t<uint32_t>& ids) {
     dst[i] = src[i] == 0xFFFFFFFFu ? -1 : int64_t(src[i]);
   return out;
 }
+
+// A captured exact search (vsg_graph_*): raw device pointers, replayed by launch().
+class PyGraph {
+ public:
+  PyGraph(py::object index, uintptr_t d_q, size_t nq, size_t k, uintptr_t d_ids, uintptr_t d_dists,
+          const std::string& method)
+      : index_(std::move(index)) {
+    auto& idx = index_.cast<GpuBruteForceIndex&>();
+    if (vsg_graph_create(idx.handle(), reinterpret_cast<const float*>(d_q), nq, k,
+                         static_cast<int>(parse_method(method)), reinterpret_cast<uint32_t*>(d_ids),
+                         reinterpret_cast<float*>(d_dists), &g_) != VSG_OK)
+      throw std::invalid_argument(vsg_last_error());
+  }
+  ~PyGraph() { close(); }
+  void launch(uintptr_t stream) {
+    if (!g_) throw std::runtime_error("graph is closed");
+    if (vsg_graph_launch(g_, reinterpret_cast<void*>(stream)) != VSG_OK) throw std::runtime_error(vsg_last_error());
+  }
+  void close() {
+    if (g_) vsg_graph_destroy(g_);
+    g_ = nullptr;
+  }
+
+ private:
+  py::object index_;
+  vsg_graph* g_ = nullptr;
+};
 
 // Python face of the serving loop (vsg_server_*): submit() keeps the request's arrays alive
 // until wait() returns them.
@@ -324,6 +352,14 @@ PYBIND11_MODULE(_vecsearch, m) {
         return self.metric() == Metric::L2 ? "l2" : "ip";
       })
       .def("__len__", &GpuBruteForceIndex::size);
+
+  py::class_<PyGraph>(m, "SearchGraph")
+      .def(py::init<py::object, uintptr_t, size_t, size_t, uintptr_t, uintptr_t, const std::string&>(),
+           py::arg("index"), py::arg("d_queries"), py::arg("nq"), py::arg("k"), py::arg("d_ids"),
+           py::arg("d_dists"), py::arg("method") = "skinny",
+           "Capture one exact search of nq device-resident queries into a HIP graph (vsg_graph_create).")
+      .def("launch", &PyGraph::launch, py::arg("stream") = 0, "Replay the captured search on a HIP stream.")
+      .def("close", &PyGraph::close);
 
   py::class_<PyServer>(m, "Server")
       .def(py::init<py::object, size_t, size_t, int, const std::string&>(), py::arg("index"),

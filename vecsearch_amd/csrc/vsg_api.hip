@@ -1403,3 +1403,93 @@ void vsg_server_destroy(vsg_server* s) {
 }
 
 }  // extern "C"
+
+// ---- captured searches -------------------------------------------------------------------
+
+struct vsg_graph {
+  vsg_index* h = nullptr;
+  Workspace w;
+  hipStream_t st = nullptr;  // capture stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t n = 0;
+  const float* base = nullptr;
+};
+
+extern "C" {
+
+int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
+                     uint32_t* d_ids, float* d_dists, vsg_graph** out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  int rc = check_search_args(h, k, method);
+  if (rc) return rc;
+  if (nq == 0 || k == 0 || !d_queries || !d_ids || !d_dists)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "a captured search needs nq > 0, k > 0 and device buffers");
+  std::lock_guard<std::mutex> g(h->mu);
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16 || method == VSG_METHOD_FP8 ||
+      (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch) && short_supported(h, k)))
+    return fail(VSG_ERR_INVALID_ARGUMENT,
+                "only the exact streaming scan can be captured (the shortlist reads its overflow count on the host)");
+  if (h->n == 0) return fail(VSG_ERR_INVALID_ARGUMENT, "cannot capture a search of an empty index");
+  VSG_HIP(hipSetDevice(h->device));
+  vsg_graph* gr = new (std::nothrow) vsg_graph;
+  if (!gr) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
+  gr->h = h;
+  gr->n = h->n;
+  gr->base = h->base;
+  auto cleanup = [&](int code) {
+    vsg_graph_destroy(gr);
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&gr->st, hipStreamNonBlocking);
+  if (e != hipSuccess) return cleanup(fail_hip("vsg_graph_create", e));
+  // size the workspace before capture (no allocation may happen inside it)
+  const ScanPlan p = plan_scan(h);
+  if ((e = gr->w.part.reserve(std::min<size_t>(kPartBudgetBytes / sizeof(uint64_t),
+                                               ((nq + 7) / 8 * 8) * size_t(p.W) * k))) != hipSuccess)
+    return cleanup(fail_hip("workspace", e));
+  e = hipStreamBeginCapture(gr->st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return cleanup(fail_hip("hipStreamBeginCapture", e));
+  const bool timing = h->timing;
+  h->timing = false;  // no event records inside the graph
+  rc = enqueue_exact(h, gr->w, d_queries, nq, k, method == VSG_METHOD_AUTO ? VSG_METHOD_SKINNY : method,
+                     nullptr, d_ids, d_dists, gr->st);
+  h->timing = timing;
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(gr->st, &graph);
+  gr->graph = graph;
+  if (rc) return cleanup(rc);
+  if (e != hipSuccess) return cleanup(fail_hip("hipStreamEndCapture", e));
+  e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return cleanup(fail_hip("hipGraphInstantiate", e));
+  *out = gr;
+  return VSG_OK;
+}
+
+int vsg_graph_launch(vsg_graph* g, void* stream) {
+  if (!g) return fail(VSG_ERR_INVALID_ARGUMENT, "null graph");
+  vsg_index* h = g->h;
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  if (h->n != g->n || h->base != g->base)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "the index changed since the graph was captured");
+  VSG_HIP(hipSetDevice(h->device));
+  VSG_HIP(hipGraphLaunch(g->exec, stream ? static_cast<hipStream_t>(stream) : h->stream));
+  return VSG_OK;
+}
+
+void vsg_graph_destroy(vsg_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->h->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->st) {
+    (void)hipStreamSynchronize(g->st);
+    (void)hipStreamDestroy(g->st);
+  }
+  g->w.release();
+  delete g;
+}
+
+}  // extern "C"
