@@ -185,6 +185,9 @@ typedef struct {
   size_t workspace_bytes;   /* admission budget of requests in flight (0: 4 GiB) */
   int slots;                /* batches in flight, one stream each (0: 2) */
   int method;               /* VSG_METHOD_* for unfiltered queries (default AUTO) */
+  int tune_mode;            /* 0: a batch whose filtered shape class is not tuned yet is tuned
+                               inline (vsg_get_param "lists_autotune"); 1: such a batch runs the
+                               planner's tile, so no batch waits for candidate runs */
 } vsg_server_config;
 
 typedef struct {

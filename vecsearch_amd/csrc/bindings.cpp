@@ -105,7 +105,7 @@ class PyGraph {
 class PyServer {
  public:
   PyServer(py::object index, size_t max_batch_queries, size_t workspace_bytes, int slots,
-           const std::string& method)
+           const std::string& method, bool tune)
       : index_(std::move(index)) {
     auto& idx = index_.cast<GpuBruteForceIndex&>();
     dim_ = idx.dim();
@@ -114,6 +114,7 @@ class PyServer {
     cfg.workspace_bytes = workspace_bytes;
     cfg.slots = slots;
     cfg.method = static_cast<int>(parse_method(method));
+    cfg.tune_mode = tune ? 0 : 1;
     if (vsg_server_create(idx.handle(), &cfg, &s_) != VSG_OK) throw std::invalid_argument(vsg_last_error());
   }
   ~PyServer() { close(); }
@@ -362,9 +363,9 @@ PYBIND11_MODULE(_vecsearch, m) {
       .def("close", &PyGraph::close);
 
   py::class_<PyServer>(m, "Server")
-      .def(py::init<py::object, size_t, size_t, int, const std::string&>(), py::arg("index"),
+      .def(py::init<py::object, size_t, size_t, int, const std::string&, bool>(), py::arg("index"),
            py::arg("max_batch_queries") = 4096, py::arg("workspace_bytes") = size_t(4) << 30,
-           py::arg("slots") = 2, py::arg("method") = "auto",
+           py::arg("slots") = 2, py::arg("method") = "auto", py::arg("tune") = true,
            "Continuous-batching request loop over an index (see vsg.h, vsg_server_*).")
       .def("submit", &PyServer::submit, py::arg("queries"), py::arg("k") = 10,
            py::arg("allow_ids") = py::none(),

@@ -673,9 +673,11 @@ int ilog2_floor(uint64_t v) {
 
 // Filtered / mixed batch through the persistent lists scan, with the tile autotuner; host
 // buffers in and out, on stream st with workspace w. The caller holds h->rw shared.
+// allow_tune = false (serving): an untuned class runs the planner's tile instead of pausing the
+// batch for the candidate runs.
 int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, size_t nq,
               const uint32_t* ks, size_t kmax, const uint64_t* range, const uint32_t* allow_ids,
-              size_t n_ids, uint32_t* out_ids, float* out_dists) {
+              size_t n_ids, uint32_t* out_ids, float* out_dists, bool allow_tune = true) {
   IndexTiming tm(h);
   const vsg::ListsTiles& fixed = h->lists_tiles;
   const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0;
@@ -707,7 +709,7 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
       return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus, st,
                                     queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists,
                                     g_err, &tm, tuned, &w.lists_last);
-    if (pairs >= (uint64_t(1) << 24)) {
+    if (allow_tune && pairs >= (uint64_t(1) << 24)) {
       static const int cand[][3] = {{8, 2, 0},    {4, 4, 0},    {4, 4, 1024}, {4, 4, 4096},
                                     {8, 2, 1024}, {2, 4, 0},    {4, 2, 1024}, {8, 4, 4096}};
       vsg::ListsTiles best;
@@ -1111,6 +1113,7 @@ struct vsg_server {
   std::unordered_map<uint64_t, std::shared_ptr<ServeRequest>> live;
   uint64_t next_ticket = 1;
   size_t inflight = 0;  // admitted estimate in flight
+  bool tune = true;     // filtered-scan tuning inside batches (off: untuned classes use the planner)
   int busy = 0;         // slots running a batch
   bool stop = false;
   vsg_server_stats stats{};
@@ -1210,7 +1213,7 @@ int run_batch(vsg_server* s, ServeSlot& sl, const std::vector<std::shared_ptr<Se
   std::vector<uint32_t> oid(nl * kl);
   std::vector<float> odist(nl * kl);
   int rc = run_lists(h, w, sl.st, q.data(), nl, ks.data(), kl, range.data(), ids.data(), nid,
-                     oid.data(), odist.data());
+                     oid.data(), odist.data(), s->tune);
   if (rc) return rc;
   at = 0;
   for (const auto& r : batch) {
@@ -1300,11 +1303,13 @@ int vsg_server_create(vsg_index* h, const vsg_server_config* cfg, vsg_server** o
   if (c.slots < 1 || c.slots > 8) return fail(VSG_ERR_INVALID_ARGUMENT, "slots must be in 1..8");
   if (c.method < VSG_METHOD_AUTO || c.method > VSG_METHOD_FP8)
     return fail(VSG_ERR_INVALID_ARGUMENT, "unknown search method");
+  if (c.tune_mode != 0 && c.tune_mode != 1) return fail(VSG_ERR_INVALID_ARGUMENT, "tune_mode must be 0 or 1");
   VSG_HIP(hipSetDevice(h->device));
   vsg_server* s = new (std::nothrow) vsg_server;
   if (!s) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
   s->h = h;
   s->cfg = c;
+  s->tune = c.tune_mode == 0;
   for (int i = 0; i < c.slots; ++i) {
     auto sl = std::make_unique<ServeSlot>();
     hipError_t e = hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking);

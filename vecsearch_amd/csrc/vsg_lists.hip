@@ -296,16 +296,16 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   // ---- plan: chunks and work items ----
   // queries per item: 4 waves x qb x dp floats of LDS must fit in 64 KB (160 KB for a
   // requested tile)
-  int qb = tiles.qb ? tiles.qb : 8, r = tiles.r;
+  int qb = tiles.qb ? tiles.qb : 4, r = tiles.r;
   const size_t lds_cap = tiles.qb ? size_t(160) << 10 : size_t(65536);
   while (qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) qb /= 2;
   if (size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) {
     err = "dim too large for the filtered scan (at most 4096)";
     return VSG_ERR_INVALID_ARGUMENT;
   }
-  if (!r || !lists_tile_supported(qb, r)) r = qb >= 4 ? 2 : 4;
+  if (!r || !lists_tile_supported(qb, r)) r = qb >= 8 ? 2 : 4;
   if (qb == 16 && r != 2) r = 2;
-  // chunk rows: aim for ~8 items per wave slot, within [256, 8192], and at most
+  // chunk rows: aim for ~2 items per wave slot, within [256, 8192], and at most
   // kMaxListsPerQuery chunks per query
   uint64_t visits = 0;
   for (const auto& kv : groups) {
@@ -313,7 +313,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     visits += (sp.second - sp.first) * ((kv.second.size() + qb - 1) / qb);
   }
   const uint64_t wave_slots = uint64_t(num_cus) * 16;
-  uint64_t C = visits / (wave_slots * 8) + 1;
+  uint64_t C = visits / (wave_slots * 2) + 1;
   C = std::min<uint64_t>(8192, std::max<uint64_t>(256, (C + 63) / 64 * 64));
   if (tiles.chunk) C = std::max<uint64_t>(64, (tiles.chunk + 63) / 64 * 64);
   uint64_t max_len = 0;
