@@ -451,7 +451,8 @@ def main():
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "first_call_s": round(tune_s, 3),
             "tile": {"qb": idx.get_param("lists_last_qb"), "r": idx.get_param("lists_last_r"),
-                     "chunk": idx.get_param("lists_last_chunk")},
+                     "chunk": idx.get_param("lists_last_chunk"),
+                     "lanes_per_row": idx.get_param("lists_last_lanes_per_row")},
             "lists_kernel_ms": round(lists_ms, 4),
             "merge_kernel_ms": round(merge_ms, 4),
             "query_row_pairs": pairs,

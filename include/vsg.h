@@ -112,15 +112,17 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
 /* Launch-shape parameters. Known names:
  *   "scan_waves_per_cu"    waves per CU of the streaming scan (1..32, default 8);
  *   "auto_short_batch"     smallest batch that AUTO sends to the shortlist (default 32);
- *   "lists_qb", "lists_r", "lists_chunk", "lists_blocks_per_cu"
+ *   "lists_qb", "lists_r", "lists_chunk", "lists_blocks_per_cu", "lists_lanes_per_row"
  *                          a fixed tile of the filtered scan (queries per work item 1/2/4/8/16,
- *                          16-row groups per lane step 2/4, allowed rows per item, blocks per
- *                          CU); 0 = chosen per call (default);
+ *                          row groups per lane step 2/4, allowed rows per item, blocks per CU,
+ *                          lanes per row 4 or 8 (8: d = 128 / 384 / 768, half the accumulator
+ *                          registers per query-row pair)); 0 = chosen per call (default);
  *   "lists_autotune"       1 (default): with no fixed tile, the first large call of each shape
  *                          class times the candidate tiles on its own inputs and the class keeps
  *                          the fastest; 0: the planner's default tile, and the table is cleared.
  * vsg_get_param reads these, and "lists_tuned_classes" (size of the tuned table) and
- * "lists_last_qb" / "lists_last_r" / "lists_last_chunk" (the tile of the last filtered call). */
+ * "lists_last_qb" / "lists_last_r" / "lists_last_chunk" / "lists_last_lanes_per_row" (the tile
+ * of the last filtered call). */
 int vsg_set_param(vsg_index* h, const char* name, int64_t value);
 int vsg_get_param(vsg_index* h, const char* name, int64_t* value);
 

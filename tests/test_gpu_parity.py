@@ -179,6 +179,28 @@ def test_search_batch_mixed_unfiltered(metric):
     np.testing.assert_array_equal(got[0][0, :ks[0]], plain[0][0])
 
 
+@pytest.mark.parametrize("d", [128, 384, 768])
+@pytest.mark.parametrize("qb,r", [(8, 4), (8, 2), (16, 2)])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_search_batch_eight_lanes_per_row(d, qb, r, metric):
+    # the eight-lanes-per-row tree (row_half_mirror, xor 2, xor 1) gives the reference's bits
+    n = 12000
+    x = dg.mixture(n, d, seed=34)
+    groups = dg.zipf_groups(n, 8, 1.1, seed=35)
+    off, ids_g = dg.group_csr(groups, 8)
+    q = dg.mixture(70, d, seed=36)
+    ks, qg = dg.query_filters(70, 8, seed=37, ks=(1, 10, 100))
+    ranges = dg.query_ranges(qg, off)
+    ranges[::5] = [vs.ALL_ROWS, 0]
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    for name, v in (("lists_qb", qb), ("lists_r", r), ("lists_chunk", 512), ("lists_lanes_per_row", 8)):
+        gpu.set_param(name, v)
+    got = gpu.search_batch(q, ks, ranges, ids_g)
+    assert gpu.get_param("lists_last_lanes_per_row") == 8
+    assert_same(got, oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=100))
+
+
 def test_search_batch_autotuner_keeps_results():
     # the first large call of a shape class times every candidate tile on its own inputs; the
     # class then reuses the fastest; results are the same bits whichever tile runs

@@ -680,7 +680,8 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
               size_t n_ids, uint32_t* out_ids, float* out_dists, bool allow_tune = true) {
   IndexTiming tm(h);
   const vsg::ListsTiles& fixed = h->lists_tiles;
-  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0;
+  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0 &&
+                       fixed.lanes_per_row == 0;
   if (planner && h->lists_autotune) {
     std::set<std::pair<uint64_t, uint64_t>> distinct;
     uint64_t pairs = 0, rows = 0;
@@ -710,15 +711,19 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
                                     queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists,
                                     g_err, &tm, tuned, &w.lists_last);
     if (allow_tune && pairs >= (uint64_t(1) << 24)) {
-      static const int cand[][3] = {{8, 2, 0},    {4, 4, 0},    {4, 4, 1024}, {4, 4, 4096},
-                                    {8, 2, 1024}, {2, 4, 0},    {4, 2, 1024}, {8, 4, 4096}};
+      // {queries per wave, row groups per lane step, chunk rows (0: planner), lanes per row}
+      static const int cand[][4] = {{4, 4, 0, 4},    {4, 4, 1024, 4}, {4, 4, 4096, 4}, {8, 2, 1024, 4},
+                                    {2, 4, 0, 4},    {8, 4, 1024, 8}, {8, 4, 4096, 8}, {8, 2, 2048, 8},
+                                    {16, 2, 2048, 8}, {16, 2, 4096, 8}};
       vsg::ListsTiles best;
       float best_ms = -1.f;
       for (const auto& c : cand) {
+        if (!vsg::lists_tile_supported(c[0], c[1], c[3], h->dim)) continue;
         vsg::ListsTiles t;
         t.qb = c[0];
         t.r = c[1];
         t.chunk = uint64_t(c[2]);
+        t.lanes_per_row = c[3];
         ScanStopwatch sw;
         vsg::ListsTiles used;
         const int rc = vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric,
@@ -959,6 +964,8 @@ int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
   } else if (key == "lists_last_qb") *value = h->ws.lists_last.qb;
   else if (key == "lists_last_r") *value = h->ws.lists_last.r;
   else if (key == "lists_last_chunk") *value = int64_t(h->ws.lists_last.chunk);
+  else if (key == "lists_last_lanes_per_row") *value = h->ws.lists_last.lanes_per_row;
+  else if (key == "lists_lanes_per_row") *value = h->lists_tiles.lanes_per_row;
   else return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
   return VSG_OK;
 }
@@ -988,6 +995,12 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
     if (value != 0 && value != 2 && value != 4)
       return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 2 or 4");
     h->lists_tiles.r = int(value);
+    return VSG_OK;
+  }
+  if (key == "lists_lanes_per_row") {
+    if (value != 0 && value != 4 && value != 8)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_lanes_per_row must be 0 (auto), 4 or 8");
+    h->lists_tiles.lanes_per_row = int(value);
     return VSG_OK;
   }
   if (key == "lists_chunk") {

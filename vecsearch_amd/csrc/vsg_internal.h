@@ -88,14 +88,15 @@ struct ListsWorkspace {
 };
 
 // Tile shape of the filtered scan (0 = chosen by the planner): queries per work item (qb),
-// row groups of 16 per lane step (r), allowed rows per item (chunk), blocks per CU.
+// row groups per lane step (r), allowed rows per item (chunk), blocks per CU, lanes per row.
 struct ListsTiles {
   int qb = 0, r = 0;
   uint64_t chunk = 0;
   int blocks_per_cu = 0;
+  int lanes_per_row = 0;  // 4 (any d) or 8 (d = 128 / 384 / 768); 0 = 4
 };
-// Supported (qb, r) pairs of the filtered scan kernel.
-bool lists_tile_supported(int qb, int r);
+// Supported (qb, r, lanes per row) tiles of the filtered scan kernel for dimension dim.
+bool lists_tile_supported(int qb, int r, int lpr, size_t dim);
 
 // Event timing of a group of launches (implemented by the index when kernel timing is on).
 struct TimingSink {

@@ -210,6 +210,58 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
   }
 }
 
+// The same distances with eight lanes per row (d a multiple of 16): each lane holds two of
+// the 16 accumulators, so a lane carries half the registers per (query, row) pair of the quad
+// layout and twice the queries or rows fit. Lane t (0..7 within its row group) owns dims
+// o(t), o(t)+1 of every 16-dim step, o(t) = 2t for t < 4 and 22 - 2t for t >= 4, so the
+// lane that DPP row_half_mirror pairs with t (7 - t) owns the accumulators j + 8 of t's
+// accumulators j. The tree of hsum256 (distance.h:18-27) then runs as:
+//   v = acc + mirror(acc)         (v_j = acc_j + acc_{j+8})
+//   w = v + xor2(v)               (w_j = v_j + v_{j+4})
+//   p = w[0] + w[1]; r = p + xor1(p)   ((w0 + w1) + (w2 + w3))
+// with every add commutative, so all eight lanes of a row hold the same bits. xr[j] points at
+// row j; q holds QB queries at stride dp in LDS.
+constexpr int kDppHalfMirror = 0x141;
+
+template <int QB, int R, int D16, bool L2>
+__device__ __forceinline__ void oct_dists(const float* const (&xr)[R], const float* q, uint32_t dp,
+                                          int t, float (&dist)[R][QB]) {
+  static_assert(D16 > 0, "eight lanes per row need d to be a multiple of 16");
+  const int o = t < 4 ? 2 * t : 22 - 2 * t;
+  float acc[R][QB][2];
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int b = 0; b < QB; ++b) acc[j][b][0] = acc[j][b][1] = 0.f;
+#pragma unroll 4
+  for (int s = 0; s < D16; ++s) {
+    float2 xv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) xv[j] = *reinterpret_cast<const float2*>(xr[j] + 16 * s + o);
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const float2 qv = *reinterpret_cast<const float2*>(q + b * dp + 16 * s + o);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        acc[j][b][0] = fma_term<L2>(qv.x, xv[j].x, acc[j][b][0]);
+        acc[j][b][1] = fma_term<L2>(qv.y, xv[j].y, acc[j][b][1]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const float v0 = acc[j][b][0] + dpp_f<kDppHalfMirror>(acc[j][b][0]);
+      const float v1 = acc[j][b][1] + dpp_f<kDppHalfMirror>(acc[j][b][1]);
+      const float w0 = v0 + dpp_f<kDppXor2>(v0);
+      const float w1 = v1 + dpp_f<kDppXor2>(v1);
+      const float p = w0 + w1;
+      const float r = p + dpp_f<kDppXor1>(p);
+      dist[j][b] = L2 ? r : 1.0f - r;
+    }
+}
+
 // Merge of the block's per-wave sorted k-lists into one sorted k-list per query, by ranking:
 // a real key's rank in the union is its slot in its own list plus, for every other list, the
 // number of keys below it (binary search). Real keys are unique, so ranks are distinct; keys

@@ -35,7 +35,8 @@ struct ListItem {
   uint32_t direct;  // 1: unfiltered group, position p is row p
 };
 
-template <int QB, int R, int D16, int KP, bool L2>
+// LPR: lanes per row, 4 (quad_dists, any d) or 8 (oct_dists, d a multiple of 16).
+template <int QB, int R, int D16, int KP, bool L2, int LPR>
 __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
     const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ queries,
     const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, uint32_t n_items,
@@ -43,7 +44,8 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
     const uint64_t* __restrict__ slot_off, uint32_t L, uint64_t* __restrict__ part,
     uint32_t* __restrict__ counter) {
   extern __shared__ float lds[];
-  const int tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
+  constexpr int kRowsPerStep = 64 / LPR;  // rows per wave instruction
+  const int tid = threadIdx.x, lane = tid & 63, t = lane & (LPR - 1), g = lane / LPR;
   float* qw = lds + (tid >> 6) * QB * dp;  // this wave's queries
 
   for (;;) {
@@ -75,19 +77,22 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
       thr[b] = kq[b] > 0 ? kPadKey : 0;  // k == 0 admits nothing
     }
 
-    for (uint32_t r0 = w.lb; r0 < w.le; r0 += 16 * R) {
+    for (uint32_t r0 = w.lb; r0 < w.le; r0 += kRowsPerStep * R) {
       uint32_t row[R];
       bool ok[R];
       const float* xr[R];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
-        const uint32_t idx = r0 + j * 16 + g;
+        const uint32_t idx = r0 + j * kRowsPerStep + g;
         ok[j] = idx < w.le;
         row[j] = w.direct ? (ok[j] ? idx : w.lb) : allow_ids[ok[j] ? idx : w.lb];
         xr[j] = base + size_t(row[j]) * dp;
       }
       float dist[R][QB];
-      quad_dists<QB, R, D16, L2, 16, kLoadPlain>(xr, ok, qw, dp, d, t, dist);
+      if constexpr (LPR == 8)
+        oct_dists<QB, R, D16, L2>(xr, qw, dp, t, dist);
+      else
+        quad_dists<QB, R, D16, L2, 16, kLoadPlain>(xr, ok, qw, dp, d, t, dist);
 #pragma unroll
       for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -158,41 +163,57 @@ struct ListsArgs {
   uint32_t* counter;
 };
 
-template <int QB, int R, int D16, int KP, bool L2>
+template <int QB, int R, int D16, int KP, bool L2, int LPR>
 hipError_t launch_lists_t(const ListsArgs& a) {
   if (a.lds > 65536) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lists_scan_kernel<QB, R, D16, KP, L2>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lists_scan_kernel<QB, R, D16, KP, L2, LPR>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(a.lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((lists_scan_kernel<QB, R, D16, KP, L2>), a.grid, dim3(kBlock), a.lds, a.st,
+  hipLaunchKernelGGL((lists_scan_kernel<QB, R, D16, KP, L2, LPR>), a.grid, dim3(kBlock), a.lds, a.st,
                      a.base, a.dp, a.d, a.q, a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off,
                      a.L, a.part, a.counter);
   return hipGetLastError();
 }
 
-template <int QB, int R, int D16>
+template <int QB, int R, int D16, int LPR>
 hipError_t launch_lists_k(bool l2, int kp, const ListsArgs& a) {
-  if (kp == 64) return l2 ? launch_lists_t<QB, R, D16, 64, true>(a) : launch_lists_t<QB, R, D16, 64, false>(a);
-  return l2 ? launch_lists_t<QB, R, D16, 128, true>(a) : launch_lists_t<QB, R, D16, 128, false>(a);
+  if (kp == 64)
+    return l2 ? launch_lists_t<QB, R, D16, 64, true, LPR>(a) : launch_lists_t<QB, R, D16, 64, false, LPR>(a);
+  return l2 ? launch_lists_t<QB, R, D16, 128, true, LPR>(a) : launch_lists_t<QB, R, D16, 128, false, LPR>(a);
 }
 
+// quad layout (4 lanes per row): any d
 template <int QB, int R>
 hipError_t launch_lists_d(bool l2, int kp, const ListsArgs& a) {
   switch (a.d) {
-    case 384: return launch_lists_k<QB, R, 24>(l2, kp, a);
-    case 768: return launch_lists_k<QB, R, 48>(l2, kp, a);
-    case 128: return launch_lists_k<QB, R, 8>(l2, kp, a);
-    default: return launch_lists_k<QB, R, 0>(l2, kp, a);
+    case 384: return launch_lists_k<QB, R, 24, 4>(l2, kp, a);
+    case 768: return launch_lists_k<QB, R, 48, 4>(l2, kp, a);
+    case 128: return launch_lists_k<QB, R, 8, 4>(l2, kp, a);
+    default: return launch_lists_k<QB, R, 0, 4>(l2, kp, a);
   }
 }
 
-hipError_t launch_lists_tile(int qb, int r, bool l2, int kp, const ListsArgs& a) {
-  if (qb == 16 && r == 2) return launch_lists_d<16, 2>(l2, kp, a);
-  if (qb == 8 && r == 4) return launch_lists_d<8, 4>(l2, kp, a);
+// eight lanes per row: d = 128, 384, 768
+template <int QB, int R>
+hipError_t launch_lists_oct(bool l2, int kp, const ListsArgs& a) {
+  switch (a.d) {
+    case 384: return launch_lists_k<QB, R, 24, 8>(l2, kp, a);
+    case 768: return launch_lists_k<QB, R, 48, 8>(l2, kp, a);
+    case 128: return launch_lists_k<QB, R, 8, 8>(l2, kp, a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_lists_tile(int qb, int r, int lpr, bool l2, int kp, const ListsArgs& a) {
+  if (lpr == 8) {
+    if (qb == 8 && r == 4) return launch_lists_oct<8, 4>(l2, kp, a);
+    if (qb == 8 && r == 2) return launch_lists_oct<8, 2>(l2, kp, a);
+    if (qb == 16 && r == 2) return launch_lists_oct<16, 2>(l2, kp, a);
+    return hipErrorInvalidValue;
+  }
   if (qb == 8 && r == 2) return launch_lists_d<8, 2>(l2, kp, a);
   if (qb == 4 && r == 4) return launch_lists_d<4, 4>(l2, kp, a);
-  if (qb == 4 && r == 2) return launch_lists_d<4, 2>(l2, kp, a);
   if (qb == 2 && r == 4) return launch_lists_d<2, 4>(l2, kp, a);
   if (qb == 1 && r == 4) return launch_lists_d<1, 4>(l2, kp, a);
   return hipErrorInvalidValue;
@@ -208,9 +229,10 @@ int hip_status(hipError_t e) {
 
 }  // namespace
 
-bool lists_tile_supported(int qb, int r) {
-  return (qb == 16 && r == 2) || (qb == 8 && (r == 2 || r == 4)) || (qb == 4 && (r == 2 || r == 4)) ||
-         (qb == 2 && r == 4) || (qb == 1 && r == 4);
+bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
+  if (lpr == 8)
+    return (dim == 128 || dim == 384 || dim == 768) && ((qb == 8 && (r == 2 || r == 4)) || (qb == 16 && r == 2));
+  return (qb == 8 && r == 2) || (qb == 4 && r == 4) || (qb == 2 && r == 4) || (qb == 1 && r == 4);
 }
 
 void ListsWorkspace::release() {
@@ -303,8 +325,13 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     err = "dim too large for the filtered scan (at most 4096)";
     return VSG_ERR_INVALID_ARGUMENT;
   }
-  if (!r || !lists_tile_supported(qb, r)) r = qb >= 8 ? 2 : 4;
-  if (qb == 16 && r != 2) r = 2;
+  int lpr = tiles.lanes_per_row == 8 ? 8 : 4;
+  if (!r) r = lpr == 8 ? (qb >= 16 ? 2 : 4) : (qb >= 8 ? 2 : 4);
+  if (!lists_tile_supported(qb, r, lpr, dim)) {  // fall back to the nearest quad tile
+    lpr = 4;
+    qb = std::min(qb, 8);
+    r = qb >= 8 ? 2 : 4;
+  }
   // chunk rows: aim for ~2 items per wave slot, within [256, 8192], and at most
   // kMaxListsPerQuery chunks per query
   uint64_t visits = 0;
@@ -326,6 +353,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   if (used) {
     used->qb = qb;
     used->r = r;
+    used->lanes_per_row = lpr;
     used->chunk = C;
     used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : 4;
   }
@@ -414,7 +442,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     const bool l2 = metric == VSG_METRIC_L2;
     const int kp = kmax <= 64 ? 64 : 128;
     if (timing) timing->begin(st);
-    LISTS_HIP(launch_lists_tile(qb, r, l2, kp, a));
+    LISTS_HIP(launch_lists_tile(qb, r, lpr, l2, kp, a));
     if (timing) timing->end(st, VSG_KERNEL_LISTS, 1);
   }
   if (timing) timing->begin(st);
