@@ -151,6 +151,24 @@ int vsg_synchronize(vsg_index* h);
  * for measurement code that needs the exact HBM footprint. */
 int vsg_base_info(const vsg_index* h, const float** d_base, size_t* padded_dim);
 
+/* ---- saved HNSW indexes (hnsw.cpp:369-420, format "VSHNSW01") ----
+ * A saved HNSWIndex holds a 52-byte header (magic, dim u64, n u64, metric u32, M u64,
+ * ef_construction u64, max_level i32, entry_point u32) followed by its n x dim fp32 vectors
+ * in id order, then the graph. vsg_hnsw_file_info reads the header; vsg_add_hnsw_file appends
+ * the vector block to an index (streamed through pinned staging into HBM, ids continuing
+ * from size()); the graph is not read. A wrong magic, a truncated file, or a dim / metric
+ * different from the index's -> INVALID. */
+int vsg_hnsw_file_info(const char* path, size_t* dim, size_t* n, int* metric);
+int vsg_add_hnsw_file(vsg_index* h, const char* path, size_t* rows_added);
+
+/* HNSWIndex::search's filter dispatch (hnsw.cpp:322-327): with a mask whose allowed count is
+ * at most exact_below, the exact scan over the allowed rows answers the batch on the GPU and
+ * *handled = 1 (no allowed row: every result missing, as hnsw.cpp:326); otherwise *handled = 0
+ * and nothing is written (the caller runs its graph search). */
+int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t k,
+                           const uint8_t* mask, size_t exact_below, uint32_t* out_ids,
+                           float* out_dists, int* handled);
+
 /* ---- captured searches (no counterpart in the reference) ----
  * vsg_graph_create records one exact search of nq device-resident queries (the streaming scan
  * and its merge, method NAIVE / SKINNY / AUTO below the shortlist batch) into a HIP graph with

@@ -236,6 +236,16 @@ class _Ref:
         idx.add(x)
         return idx.search(q, k, threads)
 
+    def hnsw_save(self, x, path, metric="l2"):
+        """HNSWIndex(x).save(path) from the reference sources (hnsw.cpp:375-389)."""
+        x = _f32(x)
+        self.lib.vsref_hnsw_save.restype = ctypes.c_int
+        self.lib.vsref_hnsw_save.argtypes = [_f32p, _sz, _sz, ctypes.c_int, ctypes.c_char_p]
+        rc = self.lib.vsref_hnsw_save(_p(x, _f32p), x.shape[0], x.shape[1], _metric(metric),
+                                      str(path).encode())
+        if rc:
+            raise RuntimeError("reference HNSWIndex::save failed")
+
     def hnsw_exact(self, x, q, k, mask, metric="l2", threads=1):
         """HNSWIndex.search(filter=mask, exact_below=len(x)): the exact filtered scan."""
         x, q = _f32(x), _f32(q)

@@ -56,4 +56,17 @@ void vsref_hnsw_filtered(const float* data, size_t n, size_t d, int metric, cons
   h.search_batch(q, nq, k, 64, ids, dists, threads, f);
 }
 
+// HNSWIndex::save (hnsw.cpp:375-389): an index of the given rows written to path, so the
+// engine's reader of that format is checked against the reference's own writer.
+int vsref_hnsw_save(const float* data, size_t n, size_t d, int metric, const char* path) {
+  try {
+    HNSWIndex h(d, n, metric == 0 ? Metric::L2 : Metric::InnerProduct);
+    h.add(data, n, 1);
+    h.save(path);
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
 }  // extern "C"

@@ -10,6 +10,9 @@ latter compared bit for bit):
   - search_d{d}.npz BruteForceIndex::search_batch (1 and 3 threads) for l2 and ip
   - filtered.npz    HNSWIndex::search(filter=mask, exact_below=n): the exact filtered scan
   - line.npz        the known answer of tests/cpp/test_brute_force.cpp:9-20
+  - hnsw_small.vshnsw / hnsw_small.npz   HNSWIndex::save of 300 x 16 rows (ip) by the reference,
+                    and those rows: the saved-index format the engine reads vectors from
+                    (`python tests/golden/make_golden.py hnsw` writes only these two)
 Run in the development container only (the reference tree is not on the GPU machines).
 """
 import os
@@ -23,8 +26,21 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 import oracle  # noqa: E402
 
 
+def hnsw_file(ref):
+    rng = np.random.default_rng(20260102)
+    x = rng.standard_normal((300, 16)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    ref.hnsw_save(x, os.path.join(HERE, "hnsw_small.vshnsw"), "ip")
+    np.savez_compressed(os.path.join(HERE, "hnsw_small.npz"), x=x)
+
+
 def main():
     ref = oracle.ref()
+    if len(sys.argv) > 1 and sys.argv[1] == "hnsw":
+        if ref is None:
+            sys.exit("oracle/_ref/libvsref.so is missing: run `make -C oracle ref` first")
+        hnsw_file(ref)
+        return
     if ref is None:
         sys.exit("oracle/_ref/libvsref.so is missing: run `make -C oracle ref` first")
     rng = np.random.default_rng(20260101)
@@ -78,6 +94,7 @@ def main():
     idx = ref.index(2, "l2")
     idx.add(pts)
     i, dd = idx.search_one(np.array([3.2, 0.0], np.float32), 3)
+    hnsw_file(ref)
     np.savez_compressed(os.path.join(HERE, "line.npz"), x=pts, q=np.array([3.2, 0.0], np.float32),
                         ids=i, dists=dd)
     print("wrote fixtures to", HERE)

@@ -30,11 +30,12 @@ except ImportError as e:  # no silent fallback: the product is the HIP engine
         "vecsearch_amd: the HIP engine is not built (run `python vecsearch_amd/build.py`): "
         f"{e}") from e
 
-from ._vecsearch import ALL_ROWS, GpuBruteForceIndex, Metric, SearchGraph, Server, cuda_available, hip_available
+from ._vecsearch import (ALL_ROWS, GpuBruteForceIndex, Metric, SearchGraph, Server, cuda_available,
+                         hip_available, hnsw_file_info)
 
 LIBRARY_PATH = os.path.join(_HERE, "libvsg_hip.so")
 
-__all__ = ["ALL_ROWS", "GpuBruteForceIndex", "Metric", "SearchGraph", "Server", "cuda_available", "hip_available", "normalize",
+__all__ = ["ALL_ROWS", "GpuBruteForceIndex", "Metric", "SearchGraph", "Server", "hnsw_file_info", "cuda_available", "hip_available", "normalize",
            "recall_at_k", "LIBRARY_PATH"]
 __version__ = "0.1.0"
 

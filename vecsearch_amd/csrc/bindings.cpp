@@ -352,7 +352,67 @@ PYBIND11_MODULE(_vecsearch, m) {
       .def_property_readonly("metric", [](const GpuBruteForceIndex& self) {
         return self.metric() == Metric::L2 ? "l2" : "ip";
       })
-      .def("__len__", &GpuBruteForceIndex::size);
+      .def("__len__", &GpuBruteForceIndex::size)
+      .def(
+          "add_hnsw_file",
+          [](GpuBruteForceIndex& self, const std::string& path) {
+            size_t rows = 0;
+            int rc;
+            {
+              py::gil_scoped_release nogil;
+              rc = vsg_add_hnsw_file(self.handle(), path.c_str(), &rows);
+            }
+            if (rc == VSG_ERR_INVALID_ARGUMENT) throw std::invalid_argument(vsg_last_error());
+            if (rc != VSG_OK) throw std::runtime_error(vsg_last_error());
+            return rows;
+          },
+          py::arg("path"),
+          "Append the vector block of a saved HNSWIndex file (hnsw.cpp:375-389) to this index;\n"
+          "returns the number of rows added. The file's dim and metric must match.")
+      .def_static(
+          "from_hnsw_file",
+          [](const std::string& path, int device) {
+            size_t dim = 0, n = 0;
+            int metric = 0;
+            if (vsg_hnsw_file_info(path.c_str(), &dim, &n, &metric) != VSG_OK)
+              throw std::invalid_argument(vsg_last_error());
+            auto idx = std::make_unique<GpuBruteForceIndex>(dim, metric == 0 ? Metric::L2 : Metric::InnerProduct,
+                                                            device);
+            int rc;
+            {
+              py::gil_scoped_release nogil;
+              rc = vsg_add_hnsw_file(idx->handle(), path.c_str(), nullptr);
+            }
+            if (rc == VSG_ERR_INVALID_ARGUMENT) throw std::invalid_argument(vsg_last_error());
+            if (rc != VSG_OK) throw std::runtime_error(vsg_last_error());
+            return idx;
+          },
+          py::arg("path"), py::arg("device") = 0,
+          "A GPU index holding the vectors of a saved HNSWIndex file (same ids, dim and metric).")
+      .def(
+          "search_exact_below",
+          [](GpuBruteForceIndex& self, FloatArray q, size_t k, const py::object& filter,
+             size_t exact_below) -> py::object {
+            if (q.ndim() == 1) q = q.reshape({py::ssize_t(1), q.shape(0)});
+            check_2d(q, self.dim());
+            const size_t nq = q.shape(0);
+            Mask mask = as_mask(filter, self.size());
+            py::array_t<uint32_t> ids({nq, k});
+            py::array_t<float> dists({nq, k});
+            int handled = 0, rc;
+            {
+              py::gil_scoped_release nogil;
+              rc = vsg_search_exact_below(self.handle(), q.data(), nq, k, mask.ptr, exact_below,
+                                          ids.mutable_data(), dists.mutable_data(), &handled);
+            }
+            if (rc == VSG_ERR_INVALID_ARGUMENT) throw std::invalid_argument(vsg_last_error());
+            if (rc != VSG_OK) throw std::runtime_error(vsg_last_error());
+            if (!handled) return py::none();
+            return py::make_tuple(ids_to_int64(ids), dists);
+          },
+          py::arg("queries"), py::arg("k") = 10, py::arg("filter") = py::none(), py::arg("exact_below") = 0,
+          "HNSWIndex.search's filter dispatch (hnsw.cpp:322-327): with a filter allowing at most\n"
+          "exact_below ids, the exact scan over them (ids, distances); otherwise None (graph search).");
 
   py::class_<PyGraph>(m, "SearchGraph")
       .def(py::init<py::object, uintptr_t, size_t, size_t, uintptr_t, uintptr_t, const std::string&>(),
@@ -378,6 +438,16 @@ PYBIND11_MODULE(_vecsearch, m) {
       .def("__exit__", [](PyServer& s, py::object, py::object, py::object) { s.close(); });
 
   m.attr("ALL_ROWS") = py::int_(static_cast<unsigned long long>(VSG_ALL_ROWS));
+  m.def(
+      "hnsw_file_info",
+      [](const std::string& path) {
+        size_t dim = 0, n = 0;
+        int metric = 0;
+        if (vsg_hnsw_file_info(path.c_str(), &dim, &n, &metric) != VSG_OK)
+          throw std::invalid_argument(vsg_last_error());
+        return py::make_tuple(dim, n, metric == 0 ? "l2" : "ip");
+      },
+      py::arg("path"), "(dim, n, metric) from the header of a saved HNSWIndex file.");
   m.def("cuda_available", [] { return cuda_available(); },
         "True if a GPU is visible (API-compatible name; the runtime is HIP).");
   m.def("hip_available", [] { return hip_available(); }, "True if a HIP GPU is visible.");

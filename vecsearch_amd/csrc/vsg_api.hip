@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <atomic>
 #include <chrono>
@@ -1508,6 +1509,124 @@ void vsg_graph_destroy(vsg_graph* g) {
   }
   g->w.release();
   delete g;
+}
+
+}  // extern "C"
+
+// ---- saved HNSW indexes and the exact_below dispatch ---------------------------------------
+
+namespace {
+
+struct HnswHeader {
+  uint64_t dim = 0, n = 0;
+  uint32_t metric = 0;
+};
+
+// hnsw.cpp:369-389: 8-byte magic, then dim, n, metric, M, ef_construction, max_level,
+// entry_point; the n x dim vector block starts at byte 52.
+constexpr uint64_t kHnswHeaderBytes = 52;
+
+int read_hnsw_header(std::ifstream& f, const char* path, HnswHeader& hd) {
+  char magic[8];
+  f.read(magic, 8);
+  uint64_t dim = 0, n = 0, M = 0, efc = 0;
+  uint32_t metric = 0, ep = 0;
+  int32_t maxl = 0;
+  f.read(reinterpret_cast<char*>(&dim), 8);
+  f.read(reinterpret_cast<char*>(&n), 8);
+  f.read(reinterpret_cast<char*>(&metric), 4);
+  f.read(reinterpret_cast<char*>(&M), 8);
+  f.read(reinterpret_cast<char*>(&efc), 8);
+  f.read(reinterpret_cast<char*>(&maxl), 4);
+  f.read(reinterpret_cast<char*>(&ep), 4);
+  if (!f) return fail(VSG_ERR_INVALID_ARGUMENT, std::string("truncated header: ") + path);
+  if (std::memcmp(magic, "VSHNSW01", 8) != 0)
+    return fail(VSG_ERR_INVALID_ARGUMENT, std::string("not a vecsearch HNSW file: ") + path);
+  if (dim == 0 || metric > 1) return fail(VSG_ERR_INVALID_ARGUMENT, std::string("bad header: ") + path);
+  hd.dim = dim;
+  hd.n = n;
+  hd.metric = metric;
+  return VSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vsg_hnsw_file_info(const char* path, size_t* dim, size_t* n, int* metric) {
+  if (!path) return fail(VSG_ERR_INVALID_ARGUMENT, "null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(VSG_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+  HnswHeader hd;
+  int rc = read_hnsw_header(f, path, hd);
+  if (rc) return rc;
+  if (dim) *dim = size_t(hd.dim);
+  if (n) *n = size_t(hd.n);
+  if (metric) *metric = int(hd.metric);
+  return VSG_OK;
+}
+
+int vsg_add_hnsw_file(vsg_index* h, const char* path, size_t* rows_added) {
+  if (!h || !path) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  if (rows_added) *rows_added = 0;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(VSG_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+  HnswHeader hd;
+  int rc = read_hnsw_header(f, path, hd);
+  if (rc) return rc;
+  if (hd.dim != h->dim) return fail(VSG_ERR_INVALID_ARGUMENT, "the file's dim differs from the index's");
+  if (int(hd.metric) != h->metric) return fail(VSG_ERR_INVALID_ARGUMENT, "the file's metric differs from the index's");
+  f.seekg(0, std::ios::end);
+  const uint64_t size = uint64_t(f.tellg());
+  if (size < kHnswHeaderBytes + hd.n * hd.dim * sizeof(float))
+    return fail(VSG_ERR_INVALID_ARGUMENT, std::string("truncated vector block: ") + path);
+  f.seekg(std::streamoff(kHnswHeaderBytes), std::ios::beg);
+  {
+    std::lock_guard<std::mutex> g(h->mu);
+    std::unique_lock<std::shared_mutex> wl(h->rw);
+    if (h->n + hd.n >= VSG_NONE) return fail(VSG_ERR_LENGTH, "an index holds at most 2^32 - 2 vectors");
+    VSG_HIP(hipSetDevice(h->device));
+    rc = grow(h, h->n + size_t(hd.n));  // one allocation for the whole block
+    if (rc) return rc;
+  }
+  // stream the block in slices of ~64 MB: read into pinned memory, then append
+  const size_t per_slice = std::max<size_t>(1, (size_t(64) << 20) / (h->dim * sizeof(float)));
+  HostBuf<float> stage;
+  hipError_t e = stage.reserve(std::min<size_t>(per_slice, std::max<uint64_t>(1, hd.n)) * h->dim);
+  if (e != hipSuccess) return fail_hip("pinned staging", e);
+  for (uint64_t r0 = 0; r0 < hd.n; r0 += per_slice) {
+    const size_t m = size_t(std::min<uint64_t>(per_slice, hd.n - r0));
+    f.read(reinterpret_cast<char*>(stage.p), std::streamsize(m * h->dim * sizeof(float)));
+    if (!f) {
+      stage.release();
+      return fail(VSG_ERR_INVALID_ARGUMENT, std::string("read failed: ") + path);
+    }
+    rc = vsg_add(h, stage.p, m);
+    if (rc) {
+      stage.release();
+      return rc;
+    }
+    if (rows_added) *rows_added += m;
+  }
+  stage.release();
+  return VSG_OK;
+}
+
+int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t k,
+                           const uint8_t* mask, size_t exact_below, uint32_t* out_ids,
+                           float* out_dists, int* handled) {
+  if (!h || !handled) return fail(VSG_ERR_INVALID_ARGUMENT, "null argument");
+  *handled = 0;
+  if (!mask) return VSG_OK;  // no filter: the graph search
+  size_t n;
+  {
+    std::shared_lock<std::shared_mutex> rl(h->rw);
+    n = h->n;
+  }
+  const size_t allowed = size_t(std::count_if(mask, mask + n, [](uint8_t v) { return v != 0; }));
+  if (allowed != 0 && allowed > exact_below) return VSG_OK;
+  *handled = 1;
+  return vsg_search(h, queries, nq, k, VSG_METHOD_AUTO, mask, out_ids, out_dists);
 }
 
 }  // extern "C"
