@@ -224,8 +224,9 @@ def serving(args, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the exact scan would shard rows across GPUs and merge top-k lists; see DESIGN.md).",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the exact scan "
+              "would shard rows across GPUs, each keeping (distance, global id) lists, and merge "
+              "the top-k lists after one all-gather; that exchange is not built here.",
               file=sys.stderr)
         sys.exit(2)
 
