@@ -118,11 +118,10 @@ struct Workspace {
   // shortlist
   DevBuf<uint8_t> s_qfrag;
   DevBuf<float> s_qscale, s_E, s_thr, s_qaux;
-  DevBuf<double> s_qnorms;
   DevBuf<uint32_t> s_tilemin, s_count, s_flags, s_rescored, s_nover;
   DevBuf<uint64_t> s_cand;
   DevBuf<uint64_t> s_emit;
-  DevBuf<uint32_t> s_emit_count, s_emit_over;
+  DevBuf<uint32_t> s_emit_count, s_slice;
   HostBuf<uint32_t> s_hover;
   DevBuf<uint32_t> f_sel;  // overflowed queries of a round (exact fallback)
   DevBuf<float> f_q;
@@ -148,7 +147,6 @@ struct Workspace {
     s_E.release();
     s_thr.release();
     s_qaux.release();
-    s_qnorms.release();
     s_tilemin.release();
     s_count.release();
     s_flags.release();
@@ -157,7 +155,7 @@ struct Workspace {
     s_cand.release();
     s_emit.release();
     s_emit_count.release();
-    s_emit_over.release();
+    s_slice.release();
     s_hover.release();
     f_sel.release();
     f_q.release();
@@ -466,7 +464,6 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   hipError_t e;
   if ((e = w.s_qfrag.reserve(qpad / 16 * ks * chunk)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = w.s_qnorms.reserve(qpad * 3)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_qaux.reserve(qpad / 16 * 32 + vsg::short_aux_pad())) != hipSuccess) return fail_hip("workspace", e);
@@ -474,7 +471,7 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   if ((e = w.s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = w.s_nover.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_nover.reserve(2)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
   // pass-2 emission regions, one per wave: a region of R rows receives about
   // round * (candidates per query) * R / n entries (rows are in no particular order); sized for
@@ -486,7 +483,7 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   const size_t n_regions = size_t(n_tiles) * vsg::short_waves();
   if ((e = w.s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = w.s_emit_over.reserve(1)) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_slice.reserve(round * vsg::short_scatter_slices())) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_hover.reserve(2)) != hipSuccess) return fail_hip("pinned staging", e);
   w.st_emitted = w.st_rescored = w.st_fallback = 0;
   w.st_nq = 0;
@@ -497,12 +494,9 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     const float* qr = d_q + r0 * h->dim;
     uint32_t* ids = d_ids + r0 * k;
     float* dists = d_dists + r0 * k;
-    VSG_HIP(vsg::launch_quantize(st, qr, uint32_t(h->dim), uint32_t(h->dim), 0, rq, uint32_t(ks), p,
-                                 w.s_qfrag.p, w.s_qscale.p, w.s_qnorms.p));
-    VSG_HIP(vsg::launch_query_bound(st, w.s_qnorms.p, rq, p, uint32_t(32 * ks), c.maxes, w.s_E.p));
-    VSG_HIP(vsg::launch_zero_u32(st, w.s_count.p, rq));
-    VSG_HIP(vsg::launch_zero_u32(st, w.s_nover.p, 1));
-    VSG_HIP(vsg::launch_zero_u32(st, w.s_emit_over.p, 1));
+    // flags: [0] queries whose candidates overflowed, [1] emission regions that overflowed
+    VSG_HIP(vsg::launch_query_prep(st, qr, uint32_t(h->dim), rq, uint32_t(ks), p, w.s_qfrag.p, w.s_qscale.p,
+                                   c.maxes, w.s_E.p, w.s_qaux.p, w.s_nover.p));
 
     vsg::ShortScanArgs a;
     a.st = st;
@@ -514,30 +508,29 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     a.qaux = w.s_qaux.p;
     a.nq = rq;
     // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk)
-    // (a looser bound only grows the candidate set, checked against its capacity)
-    const uint32_t tstride = uint32_t(std::min<size_t>(16, std::max<size_t>(1, n_tiles / (4 * k))));
+    // (a looser bound only grows the candidate set, checked against its capacity): at least 4k
+    // tiles and about 256 (one per CU, the pass costs one block's time), at most every 16th
+    const uint32_t tstride = uint32_t(std::max<size_t>(
+        1, std::min<size_t>({16, n_tiles / (4 * k), (size_t(n_tiles) + 255) / 256})));
     const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
     a.tilemin = w.s_tilemin.p;
     a.emit = w.s_emit.p;
     a.wave_cap = wave_cap;
     a.emit_count = w.s_emit_count.p;
-    a.overflow = w.s_emit_over.p;
+    a.overflow = w.s_nover.p + 1;
     a.cand = w.s_cand.p;
     a.cap = cap;
     a.count = w.s_count.p;
-    VSG_HIP(vsg::launch_query_aux(st, nullptr, p == 2 ? w.s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
-                                  w.s_qaux.p));
+    a.slice = w.s_slice.p;
     {
       TimedScope ts(h, st, VSG_KERNEL_SAMPLE);
       a.tstride = tstride;
       a.n_space = a.n_blocks = space1;
       VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
-      VSG_HIP(vsg::launch_select_threshold(st, w.s_tilemin.p, space1, rq, nullptr, uint32_t(k), w.s_E.p,
-                                           w.s_thr.p));
+      VSG_HIP(vsg::launch_select_threshold(st, w.s_tilemin.p, space1, rq, uint32_t(qpad), nullptr, uint32_t(k),
+                                           w.s_E.p, w.s_qscale.p, w.s_thr.p, w.s_qaux.p));
       ts.launches++;
     }
-    VSG_HIP(vsg::launch_query_aux(st, w.s_thr.p, p == 2 ? w.s_qscale.p : nullptr, rq, uint32_t(qpad / 16),
-                                  w.s_qaux.p));
     {
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
       a.tstride = 1;
@@ -556,8 +549,7 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
                                   dists, uint32_t(k), w.s_flags.p, w.s_nover.p, w.s_rescored.p));
       ts.launches++;
     }
-    VSG_HIP(hipMemcpyAsync(w.s_hover.p, w.s_nover.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    VSG_HIP(hipMemcpyAsync(w.s_hover.p + 1, w.s_emit_over.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VSG_HIP(hipMemcpyAsync(w.s_hover.p, w.s_nover.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VSG_HIP(hipStreamSynchronize(st));
     if (w.s_hover.p[1]) {  // an emission region overflowed: answer this round exactly
       w.st_fallback += rq;
@@ -1143,7 +1135,7 @@ size_t request_bytes(const vsg_index* h, size_t nq, size_t k, bool filtered, siz
   const size_t io = nq * (h->dim * 4 + k * 8) * 2;
   if (!filtered) {
     const size_t tiles = h->n / 256 + 1;
-    return io + nq * (size_t(vsg::short_cand_cap()) * 8 + tiles * 4 + 64 * k * 8);
+    return io + nq * (size_t(vsg::short_cand_cap()) * 8 + vsg::short_scatter_slices() * 4 + tiles * 4 + 64 * k * 8);
   }
   const size_t chunks = n_allow / 1024 + 1;
   return io + n_allow * 4 + nq * chunks * k * 8;

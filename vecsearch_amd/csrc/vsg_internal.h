@@ -50,27 +50,31 @@ struct ShortScanArgs {
   uint32_t cap;
   uint32_t* count;
   uint32_t* overflow;     // blocks whose region overflowed
+  uint32_t* slice;        // nq * short_scatter_slices() counters of the scatter
 };
+size_t short_scatter_slices();
 int short_ks(uint32_t d);
 int short_cand_cap();
 int short_tile_rows_for(int prec, int ks);
 int short_group(int prec, int ks);
 int short_aux_pad();
-hipError_t launch_query_aux(hipStream_t st, const float* thr, const float* qscale, uint32_t nq,
-                            uint32_t nfrag_pad, float* aux);
 hipError_t launch_quantize(hipStream_t st, const float* src, uint32_t ld, uint32_t d, uint32_t r0,
                            uint32_t r1, uint32_t KS, int prec, void* out, float* scale,
                            double* norms);
 hipError_t launch_norm_max(hipStream_t st, const double* norms, uint32_t r0, uint32_t r1,
                            unsigned long long* maxes);
-hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq, int prec, uint32_t d,
-                              const unsigned long long* maxes, float* E);
+// queries [0, nq): fragments (out), scales, error bounds E, the pass-1 aux block; flags[0..1] = 0
+hipError_t launch_query_prep(hipStream_t st, const float* queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
+                             void* out, float* qscale, const unsigned long long* maxes, float* E, float* aux,
+                             uint32_t* flags);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a);
 hipError_t launch_scatter(int prec, const ShortScanArgs& a);
 int short_waves();  // waves per shortlist scan block (emission regions per tile)
+// thresholds of queries [0, nq) and the pass-2 aux block of queries [0, nq_pad) (padding: +inf)
 hipError_t launch_select_threshold(hipStream_t st, const uint32_t* tilemin, uint32_t n_tiles, uint32_t nq,
-                                   const uint32_t* ks, uint32_t k, const float* E, float* thr);
+                                   uint32_t nq_pad, const uint32_t* ks, uint32_t k, const float* E,
+                                   const float* qscale, float* thr, float* aux);
 hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t dp, uint32_t d,
                           const float* queries, uint32_t nq, const uint32_t* ks, uint32_t k,
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,

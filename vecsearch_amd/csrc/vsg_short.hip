@@ -57,26 +57,22 @@ __device__ __forceinline__ float from_e4m3(uint8_t b) {
   return __builtin_amdgcn_cvt_f32_fp8(int(b), 0);
 }
 
-// One wave per row (or query): writes the row's fragment slots and returns, in lane 0,
-// |x|^2, |x^|^2 and |x - x^|^2 (FP64) and for FP8 the scale.
-// src: row-major fp32, stride ld, d valid dims; rows [r0, r1); KS k-steps of 32 dims.
-// prec 1 = bf16 -> out16, prec 2 = fp8 -> out8 + scale.
-__global__ __launch_bounds__(256) void quantize_kernel(
-    const float* __restrict__ src, uint32_t ld, uint32_t d, uint32_t r0, uint32_t r1, uint32_t KS,
-    int prec, __bf16* __restrict__ out16, uint8_t* __restrict__ out8, float* __restrict__ scale,
-    double* __restrict__ norms /* per row: |x|^2, |x^|^2, |x - x^|^2 */) {
+// One wave per row (or query): writes the row's fragment slots and returns, in every lane,
+// |x|^2, |x^|^2 and |x - x^|^2 (FP64) and for FP8 the scale sx (1 for BF16).
+// x: the row's d valid fp32 dims; r: its fragment row; KS k-steps of 32 dims.
+// prec 1 = bf16 -> out16, prec 2 = fp8 -> out8.
+__device__ __forceinline__ void quantize_row(const float* __restrict__ x, uint32_t d, uint32_t r, uint32_t KS,
+                                             int prec, __bf16* __restrict__ out16, uint8_t* __restrict__ out8,
+                                             double& s_x, double& s_h, double& s_e, float& sx) {
   const int lane = threadIdx.x & 63;
-  const uint32_t r = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= r1) return;
-  const float* x = src + size_t(r) * ld;
-  float sx = 1.f;
+  sx = 1.f;
   if (prec == 2) {
     float mx = 0.f;
     for (uint32_t j = lane; j < d; j += 64) mx = fmaxf(mx, fabsf(x[j]));
     for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
     sx = mx > 0.f ? mx / 448.f : 1.f;
   }
-  double s_x = 0, s_h = 0, s_e = 0;
+  s_x = 0, s_h = 0, s_e = 0;
   const uint32_t groups = KS * 4;  // 8-dim groups
   for (uint32_t gi = lane; gi < groups; gi += 64) {
     const uint32_t s = gi >> 2, g = gi & 3;
@@ -106,7 +102,20 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     s_h += __shfl_xor(s_h, off, 64);
     s_e += __shfl_xor(s_e, off, 64);
   }
-  if (lane == 0) {
+}
+
+// Base rows [r0, r1) of a row-major fp32 matrix (stride ld): fragments, per-row norms
+// (|x|^2, |x^|^2, |x - x^|^2) and, for FP8, per-row scales.
+__global__ __launch_bounds__(256) void quantize_kernel(
+    const float* __restrict__ src, uint32_t ld, uint32_t d, uint32_t r0, uint32_t r1, uint32_t KS,
+    int prec, __bf16* __restrict__ out16, uint8_t* __restrict__ out8, float* __restrict__ scale,
+    double* __restrict__ norms) {
+  const uint32_t r = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= r1) return;
+  double s_x, s_h, s_e;
+  float sx;
+  quantize_row(src + size_t(r) * ld, d, r, KS, prec, out16, out8, s_x, s_h, s_e, sx);
+  if ((threadIdx.x & 63) == 0) {
     norms[size_t(r) * 3 + 0] = s_x;
     norms[size_t(r) * 3 + 1] = s_h;
     norms[size_t(r) * 3 + 2] = s_e;
@@ -128,17 +137,14 @@ __global__ void norm_max_kernel(const double* __restrict__ norms, uint32_t r0, u
 }
 
 // Per-query error bound E (see the header) from the query norms and the base maxima.
-__global__ void query_bound_kernel(const double* __restrict__ qnorms, uint32_t nq, int prec,
-                                   uint32_t d, const unsigned long long* __restrict__ maxes,
-                                   float* __restrict__ E) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+__device__ __forceinline__ float query_error_bound(double qx2, double qh2, double qe2, int prec, uint32_t d,
+                                                   const unsigned long long* __restrict__ maxes) {
   const double X = __longlong_as_double((long long)maxes[0]);
   const double Xh = __longlong_as_double((long long)maxes[1]);
   const double dX = __longlong_as_double((long long)maxes[2]);
-  const double nq_ = sqrt(qnorms[size_t(q) * 3 + 0]);
-  const double nqh = sqrt(qnorms[size_t(q) * 3 + 1]);
-  const double dq = sqrt(qnorms[size_t(q) * 3 + 2]);
+  const double nq_ = sqrt(qx2);
+  const double nqh = sqrt(qh2);
+  const double dq = sqrt(qe2);
   const double u = 1.0 / 16777216.0;
   const double dd = double(d), m = ceil(dd / 16.0) + 12.0;  // d: the padded width 32 * KS
   const double gd = dd * u / (1 - dd * u), gm = m * u / (1 - m * u);
@@ -147,7 +153,27 @@ __global__ void query_bound_kernel(const double* __restrict__ qnorms, uint32_t n
   if (prec == 2) e += 4.0 * u * nqh * Xh;
   e += gm * nq_ * X;
   e += 4.0 * u * (1.0 + nq_ * X);
-  E[q] = float(e * (1.0 + 1e-6) + 1e-12);
+  return float(e * (1.0 + 1e-6) + 1e-12);
+}
+
+// The first launch of a shortlist round, one wave per query: query fragments, E[q], the FP8
+// scale, the scale slots of the pass-1 aux block (pass 1 reads no cuts), and flags[0..1] = 0.
+__global__ __launch_bounds__(256) void query_prep_kernel(
+    const float* __restrict__ queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
+    __bf16* __restrict__ out16, uint8_t* __restrict__ out8, float* __restrict__ qscale,
+    const unsigned long long* __restrict__ maxes, float* __restrict__ E, float* __restrict__ aux,
+    uint32_t* __restrict__ flags) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) flags[threadIdx.x] = 0;
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  double s_x, s_h, s_e;
+  float sx;
+  quantize_row(queries + size_t(q) * d, d, q, KS, prec, out16, out8, s_x, s_h, s_e, sx);
+  if ((threadIdx.x & 63) == 0) {
+    E[q] = query_error_bound(s_x, s_h, s_e, prec, 32 * KS, maxes);
+    qscale[q] = sx;
+    aux[(q >> 4) * 32 + 16 + (q & 15)] = sx;
+  }
 }
 
 // ---- the MFMA scan (rows stationary, queries streaming) ---------------------------------
@@ -424,26 +450,97 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 }
 
 // Files the pass-2 entries of every wave region into per-query candidate lists (count[q] keys
-// (ord(dh) << 32) | row at cand[q * cap ..]); a region that overflowed sets *overflow (the round
-// is then answered by the exact scan). One block of 256 threads per tile of 8 wave regions;
-// region r covers rows r * rows_per_wave ...
-__global__ __launch_bounds__(256) void scatter_candidates_kernel(
+// (ord(dh) << 32) | row at cand[q * cap ..]) without global atomics on the per-query counters
+// (device-scope atomics on a few hundred hot addresses serialise); region r covers rows
+// r * rows_per_wave ... Three launches over S slices of `per` consecutive regions each:
+//   count   per slice, entries per query in LDS -> slice_cnt[q * kScatterSlices + s]; a region that
+//           overflowed sets *overflow (the round is then answered by the exact scan);
+//   offsets per query (one wave), exclusive scan over the slices in place, total -> count[q];
+//   place   per slice, each entry at cand[q * cap + offset + its LDS rank] if below cap.
+// The order of a query's candidates is not fixed; the rescore's result does not depend on it.
+constexpr uint32_t kScatterSlices = 1024;
+
+__global__ __launch_bounds__(256) void scatter_count_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
-    uint32_t rows_per_wave, uint64_t* __restrict__ cand, uint32_t cap, uint32_t* __restrict__ count,
+    uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S, uint32_t* __restrict__ slice_cnt,
     uint32_t* __restrict__ overflow) {
-  for (int w = 0; w < kWaves; ++w) {
-    const size_t r = size_t(blockIdx.x) * kWaves + w;
-    const uint32_t c = emit_count[r];
-    if (c > wave_cap) {
-      if (threadIdx.x == 0) atomicAdd(overflow, 1u);
-      continue;
+  extern __shared__ uint32_t hist[];  // nq
+  __shared__ uint32_t rc[64];
+  const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t r0 = s * per, r1 = min(n_regions, r0 + per);
+  for (uint32_t q = tid; q < nq; q += 256) hist[q] = 0;
+  for (uint32_t rb = r0; rb < r1; rb += 64) {  // region counts staged 64 at a time
+    __syncthreads();
+    if (tid < min(64u, r1 - rb)) rc[tid] = emit_count[rb + tid];
+    __syncthreads();
+    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += 4) {
+      const uint32_t c = rc[r - rb];
+      if (c > wave_cap) {
+        if (lane == 0) atomicAdd(overflow, 1u);
+        continue;
+      }
+      for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[uint32_t(emit[size_t(r) * wave_cap + i]) >> 6], 1u);
     }
-    for (uint32_t i = threadIdx.x; i < c; i += 256) {
-      const uint64_t e = emit[r * wave_cap + i];
-      const uint32_t lo = uint32_t(e), q = lo >> 6;
-      const uint32_t row = uint32_t(r) * rows_per_wave + (lo & 63);
-      const uint32_t slot = atomicAdd(count + q, 1u);
-      if (slot < cap) cand[size_t(q) * cap + slot] = (e & 0xFFFFFFFF00000000ull) | row;
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < nq; q += 256) slice_cnt[size_t(q) * kScatterSlices + s] = hist[q];
+}
+
+// slice_cnt rows have kScatterSlices entries (16-byte aligned); each lane scans 16 of them
+__global__ __launch_bounds__(256) void scatter_offsets_kernel(uint32_t* __restrict__ slice_cnt, uint32_t S,
+                                                               uint32_t nq, uint32_t* __restrict__ count) {
+  static_assert(kScatterSlices == 64 * 16, "one wave of 16 slices a lane");
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= nq) return;
+  uint4* v = reinterpret_cast<uint4*>(slice_cnt + size_t(q) * kScatterSlices + lane * 16);
+  uint32_t x[16];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint4 t = v[j];
+    x[4 * j] = t.x, x[4 * j + 1] = t.y, x[4 * j + 2] = t.z, x[4 * j + 3] = t.w;
+  }
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {  // exclusive offsets within the lane; slices >= S hold nothing
+    const uint32_t c = lane * 16 + j < S ? x[j] : 0u;
+    x[j] = sum;
+    sum += c;
+  }
+  uint32_t incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= uint32_t(off)) incl += y;
+  }
+  const uint32_t base = incl - sum;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    v[j] = make_uint4(base + x[4 * j], base + x[4 * j + 1], base + x[4 * j + 2], base + x[4 * j + 3]);
+  if (lane == 63) count[q] = incl;
+}
+
+__global__ __launch_bounds__(256) void scatter_place_kernel(
+    const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
+    uint32_t rows_per_wave, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
+    const uint32_t* __restrict__ slice_off, uint64_t* __restrict__ cand, uint32_t cap) {
+  extern __shared__ uint32_t base[];  // nq
+  __shared__ uint32_t rc[64];
+  const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t r0 = s * per, r1 = min(n_regions, r0 + per);
+  for (uint32_t q = tid; q < nq; q += 256) base[q] = slice_off[size_t(q) * kScatterSlices + s];
+  for (uint32_t rb = r0; rb < r1; rb += 64) {
+    __syncthreads();
+    if (tid < min(64u, r1 - rb)) rc[tid] = emit_count[rb + tid];
+    __syncthreads();
+    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += 4) {
+      const uint32_t c = rc[r - rb];
+      if (c > wave_cap) continue;
+      for (uint32_t i = lane; i < c; i += 64) {
+        const uint64_t e = emit[size_t(r) * wave_cap + i];
+        const uint32_t lo = uint32_t(e), q = lo >> 6;
+        const uint32_t slot = atomicAdd(&base[q], 1u);
+        if (slot < cap) cand[size_t(q) * cap + slot] = (e & 0xFFFFFFFF00000000ull) | (r * rows_per_wave + (lo & 63));
+      }
     }
   }
 }
@@ -467,31 +564,86 @@ __device__ __forceinline__ float score_cut(float T) {
   return round_down_f(1.0 - t - fabs(t) * 0x1p-22 - 0x1p-60);
 }
 
-// Builds the per-fragment auxiliary block of the query stages: for every fragment f,
-// aux[f * 32 + i] = score_cut(thr[q]) and aux[f * 32 + 16 + i] = qscale[q] (q = 16 f + i;
-// padding queries +inf / 1).
-__global__ void query_aux_kernel(const float* __restrict__ thr, const float* __restrict__ qscale,
-                                 uint32_t nq, uint32_t nfrag_pad, float* __restrict__ aux) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nfrag_pad * 16) return;
-  const uint32_t f = t >> 4, i = t & 15, q = t;
-  aux[f * 32 + i] = (thr && q < nq) ? score_cut(thr[q]) : __builtin_inff();
-  aux[f * 32 + 16 + i] = (qscale && q < nq) ? qscale[q] : 1.f;
+// Pass-2 aux block of the query stages: for query q = 16 f + i, aux[f * 32 + i] = its score cut
+// (score_cut of its threshold; +inf for padding queries, which then never emit) and
+// aux[f * 32 + 16 + i] = its scale (1 for padding queries).
+__device__ __forceinline__ void set_pass2_aux(float* aux, uint32_t q, float cut, float sc) {
+  aux[(q >> 4) * 32 + (q & 15)] = cut;
+  aux[(q >> 4) * 32 + 16 + (q & 15)] = sc;
 }
 
 // ---- threshold select --------------------------------------------------------------------
 // Per query (one 256-thread block): U = the kq-th smallest tile minimum (radix select on the
 // order-preserving bits), threshold = U + 2E. With fewer tiles than kq the threshold is +inf.
+// The same selection by one wave per query when the tile count is at most 64 * NJ (NJ <= 16):
+// each lane holds NJ minima and the kq-th smallest is found bit by bit with ballot counts.
+template <int NJ>
+__global__ __launch_bounds__(256) void select_threshold_wave_kernel(
+    const uint32_t* __restrict__ tilemin, uint32_t n_tiles, uint32_t nq, uint32_t nq_pad,
+    const uint32_t* __restrict__ ks, uint32_t k, const float* __restrict__ E, const float* __restrict__ qscale,
+    float* __restrict__ thr, float* __restrict__ aux) {
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= nq_pad) return;
+  if (q >= nq) {
+    if (lane == 0) set_pass2_aux(aux, q, __builtin_inff(), 1.f);
+    return;
+  }
+  const uint32_t kq = ks ? ks[q] : k;
+  if (kq == 0 || kq > n_tiles) {
+    if (lane == 0) {
+      thr[q] = __builtin_inff();
+      set_pass2_aux(aux, q, score_cut(__builtin_inff()), qscale[q]);
+    }
+    return;
+  }
+  const uint32_t* v = tilemin + size_t(q) * n_tiles;
+  uint32_t key[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t i = lane + 64 * j;
+    key[j] = i < n_tiles ? v[i] : 0xFFFFFFFFu;
+  }
+  uint32_t prefix = 0, rank = kq;  // find the rank-th smallest (1-based) among the keys
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t hi_mask = b == 31 ? 0u : (0xFFFFFFFFu << (b + 1));
+    uint32_t cnt = 0;  // keys matching the prefix above bit b with bit b clear
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const uint32_t i = lane + 64 * j;
+      const bool m = i < n_tiles && (key[j] & hi_mask) == (prefix & hi_mask) && !((key[j] >> b) & 1u);
+      cnt += uint32_t(__builtin_popcountll(ballot(m)));
+    }
+    if (cnt < rank) {
+      rank -= cnt;
+      prefix |= 1u << b;
+    }
+  }
+  if (lane == 0) {
+    const float U = unord_bits(prefix);
+    const float t = round_up_f(double(U) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    thr[q] = t;
+    set_pass2_aux(aux, q, score_cut(t), qscale[q]);
+  }
+}
+
 __global__ __launch_bounds__(256) void select_threshold_kernel(
-    const uint32_t* __restrict__ tilemin, uint32_t n_tiles, const uint32_t* __restrict__ ks,
-    uint32_t k, const float* __restrict__ E, float* __restrict__ thr) {
+    const uint32_t* __restrict__ tilemin, uint32_t n_tiles, uint32_t nq, const uint32_t* __restrict__ ks,
+    uint32_t k, const float* __restrict__ E, const float* __restrict__ qscale, float* __restrict__ thr,
+    float* __restrict__ aux) {
   __shared__ uint32_t hist[256];
   __shared__ uint32_t sel[2];
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  if (q >= nq) {
+    if (tid == 0) set_pass2_aux(aux, q, __builtin_inff(), 1.f);
+    return;
+  }
   const uint32_t kq = ks ? ks[q] : k;
   const uint32_t* v = tilemin + size_t(q) * n_tiles;
   if (kq == 0 || kq > n_tiles) {
-    if (tid == 0) thr[q] = __builtin_inff();
+    if (tid == 0) {
+      thr[q] = __builtin_inff();
+      set_pass2_aux(aux, q, score_cut(__builtin_inff()), qscale[q]);
+    }
     return;
   }
   uint32_t prefix = 0, pmask = 0, rank = kq;  // 1-based rank within the current prefix
@@ -521,7 +673,9 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
   if (tid == 0) {
     const float U = unord_bits(prefix);
     // U + 2E (with a relative cushion on E and an absolute one of 2^-20), rounded up
-    thr[q] = round_up_f(double(U) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    const float t = round_up_f(double(U) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    thr[q] = t;
+    set_pass2_aux(aux, q, score_cut(t), qscale[q]);
   }
 }
 
@@ -667,10 +821,12 @@ hipError_t launch_norm_max(hipStream_t st, const double* norms, uint32_t r0, uin
   return hipGetLastError();
 }
 
-hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq, int prec, uint32_t d,
-                              const unsigned long long* maxes, float* E) {
-  hipLaunchKernelGGL(query_bound_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, qnorms, nq, prec, d,
-                     maxes, E);
+hipError_t launch_query_prep(hipStream_t st, const float* queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
+                             void* out, float* qscale, const unsigned long long* maxes, float* E, float* aux,
+                             uint32_t* flags) {
+  hipLaunchKernelGGL(query_prep_kernel, dim3((std::max(nq, 1u) + 3) / 4), dim3(256), 0, st, queries, d, nq, KS, prec,
+                     prec == 1 ? static_cast<__bf16*>(out) : nullptr,
+                     prec == 2 ? static_cast<uint8_t*>(out) : nullptr, qscale, maxes, E, aux, flags);
   return hipGetLastError();
 }
 
@@ -679,20 +835,25 @@ hipError_t launch_query_bound(hipStream_t st, const double* qnorms, uint32_t nq,
 int short_group(int prec, int ks) { return stage_group(prec, ks); }
 int short_aux_pad() { return 64; }
 
-hipError_t launch_query_aux(hipStream_t st, const float* thr, const float* qscale, uint32_t nq,
-                            uint32_t nfrag_pad, float* aux) {
-  hipLaunchKernelGGL(query_aux_kernel, dim3((nfrag_pad * 16 + 255) / 256), dim3(256), 0, st, thr, qscale, nq,
-                     nfrag_pad, aux);
-  return hipGetLastError();
-}
-
 int short_waves() { return kWaves; }
 
 hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
-  hipLaunchKernelGGL(scatter_candidates_kernel, dim3(a.n_blocks), dim3(256), 0, a.st, a.emit, a.emit_count,
-                     a.wave_cap, uint32_t(rows_frags(prec, a.KS) * 16), a.cand, a.cap, a.count, a.overflow);
+  const uint32_t n_regions = a.n_blocks * uint32_t(kWaves);
+  if (n_regions == 0 || a.nq == 0) return hipErrorInvalidValue;  // count[] is written only below
+  // about 16 regions a slice (4 a wave), at most kScatterSlices slices
+  const uint32_t S = std::min(kScatterSlices, std::max(1u, (n_regions + 15) / 16));
+  const uint32_t per = (n_regions + S - 1) / S;
+  const size_t lds = size_t(a.nq) * 4;
+  hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap,
+                     n_regions, per, a.nq, S, a.slice, a.overflow);
+  hipLaunchKernelGGL(scatter_offsets_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, a.st, a.slice, S, a.nq,
+                     a.count);
+  hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap,
+                     uint32_t(rows_frags(prec, a.KS) * 16), n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
   return hipGetLastError();
 }
+
+size_t short_scatter_slices() { return kScatterSlices; }
 
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   if (!n) return hipSuccess;
@@ -748,8 +909,24 @@ hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a) {
 }
 
 hipError_t launch_select_threshold(hipStream_t st, const uint32_t* tilemin, uint32_t n_tiles, uint32_t nq,
-                                   const uint32_t* ks, uint32_t k, const float* E, float* thr) {
-  hipLaunchKernelGGL(select_threshold_kernel, dim3(nq), dim3(256), 0, st, tilemin, n_tiles, ks, k, E, thr);
+                                   uint32_t nq_pad, const uint32_t* ks, uint32_t k, const float* E,
+                                   const float* qscale, float* thr, float* aux) {
+  const dim3 wg((nq_pad + 3) / 4), b(256);
+  if (n_tiles <= 128)
+    hipLaunchKernelGGL(select_threshold_wave_kernel<2>, wg, b, 0, st, tilemin, n_tiles, nq, nq_pad, ks, k, E, qscale,
+                       thr, aux);
+  else if (n_tiles <= 256)
+    hipLaunchKernelGGL(select_threshold_wave_kernel<4>, wg, b, 0, st, tilemin, n_tiles, nq, nq_pad, ks, k, E, qscale,
+                       thr, aux);
+  else if (n_tiles <= 512)
+    hipLaunchKernelGGL(select_threshold_wave_kernel<8>, wg, b, 0, st, tilemin, n_tiles, nq, nq_pad, ks, k, E, qscale,
+                       thr, aux);
+  else if (n_tiles <= 1024)
+    hipLaunchKernelGGL(select_threshold_wave_kernel<16>, wg, b, 0, st, tilemin, n_tiles, nq, nq_pad, ks, k, E,
+                       qscale, thr, aux);
+  else
+    hipLaunchKernelGGL(select_threshold_kernel, dim3(nq_pad), b, 0, st, tilemin, n_tiles, nq, ks, k, E, qscale, thr,
+                       aux);
   return hipGetLastError();
 }
 
