@@ -180,10 +180,11 @@ def test_search_batch_mixed_unfiltered(metric):
 
 
 @pytest.mark.parametrize("d", [128, 384, 768])
-@pytest.mark.parametrize("qb,r", [(8, 4), (8, 2), (16, 2)])
+@pytest.mark.parametrize("qb,r", [(8, 4), (8, 2), (16, 2), (32, 8)])
 @pytest.mark.parametrize("metric", ["l2", "ip"])
 def test_search_batch_eight_lanes_per_row(d, qb, r, metric):
-    # the eight-lanes-per-row tree (row_half_mirror, xor 2, xor 1) gives the reference's bits
+    # the eight-lanes-per-row tree (row_half_mirror, xor 2, xor 1) gives the reference's bits;
+    # qb = 32: one item per block, rows and query dims streamed through the LDS ring
     n = 12000
     x = dg.mixture(n, d, seed=34)
     groups = dg.zipf_groups(n, 8, 1.1, seed=35)
@@ -219,7 +220,7 @@ def test_search_batch_autotuner_keeps_results():
     assert_same(first, oracle.search_lists(x, q, ks, ranges, allow, "ip", threads=8, kmax=10))
     second = gpu.search_batch(q, ks, ranges, allow)
     assert_same(second, (np.where(first[0] < 0, oracle.NONE, first[0]).astype(np.uint32), first[1]))
-    assert gpu.get_param("lists_last_qb") in (1, 2, 4, 8, 16)
+    assert gpu.get_param("lists_last_qb") in (1, 2, 4, 8, 16, 32)
     gpu.set_param("lists_autotune", 0)
     assert gpu.get_param("lists_tuned_classes") == 0
     gpu.set_param("lists_qb", 4)

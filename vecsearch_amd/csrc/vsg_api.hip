@@ -705,9 +705,11 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
                                     g_err, &tm, tuned, &w.lists_last);
     if (allow_tune && pairs >= (uint64_t(1) << 24)) {
       // {queries per wave, row groups per lane step, chunk rows (0: planner), lanes per row}
-      static const int cand[][4] = {{4, 4, 0, 4},    {4, 4, 1024, 4}, {4, 4, 4096, 4}, {8, 2, 1024, 4},
-                                    {2, 4, 0, 4},    {8, 4, 1024, 8}, {8, 4, 4096, 8}, {8, 2, 2048, 8},
-                                    {16, 2, 2048, 8}, {16, 2, 4096, 8}};
+      // (32 queries: one item per block, every row read shared by its four waves)
+      static const int cand[][4] = {{4, 4, 0, 4},     {4, 4, 1024, 4},  {4, 4, 4096, 4}, {8, 2, 1024, 4},
+                                    {2, 4, 0, 4},     {8, 4, 1024, 8},  {8, 4, 4096, 8}, {8, 2, 2048, 8},
+                                    {16, 2, 2048, 8}, {16, 2, 4096, 8}, {32, 8, 0, 8},   {32, 8, 2048, 8},
+                                    {32, 8, 8192, 8}};
       vsg::ListsTiles best;
       float best_ms = -1.f;
       for (const auto& c : cand) {
@@ -979,14 +981,14 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
     return VSG_OK;
   }
   if (key == "lists_qb") {
-    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
-      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_qb must be 0 (auto), 1, 2, 4, 8 or 16");
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_qb must be 0 (auto), 1, 2, 4, 8, 16 or 32");
     h->lists_tiles.qb = int(value);
     return VSG_OK;
   }
   if (key == "lists_r") {
-    if (value != 0 && value != 2 && value != 4)
-      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 2 or 4");
+    if (value != 0 && value != 2 && value != 4 && value != 8)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 2, 4 or 8");
     h->lists_tiles.r = int(value);
     return VSG_OK;
   }

@@ -223,6 +223,20 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
 // row j; q holds QB queries at stride dp in LDS.
 constexpr int kDppHalfMirror = 0x141;
 
+// The distance of one query-row pair from its eight lanes' accumulators (a0: dim o(t),
+// a1: dim o(t) + 1, lane t = lane & 7 of the row's group): the accumulator tree of hsum256
+// (distance.h:18-27) with DPP exchanges, left in every lane of the group.
+template <bool L2>
+__device__ __forceinline__ float oct_reduce(float a0, float a1) {
+  const float v0 = a0 + dpp_f<kDppHalfMirror>(a0);
+  const float v1 = a1 + dpp_f<kDppHalfMirror>(a1);
+  const float w0 = v0 + dpp_f<kDppXor2>(v0);
+  const float w1 = v1 + dpp_f<kDppXor2>(v1);
+  const float p = w0 + w1;
+  const float r = p + dpp_f<kDppXor1>(p);
+  return L2 ? r : 1.0f - r;
+}
+
 template <int QB, int R, int D16, bool L2>
 __device__ __forceinline__ void oct_dists(const float* const (&xr)[R], const float* q, uint32_t dp,
                                           int t, float (&dist)[R][QB]) {
@@ -251,16 +265,43 @@ __device__ __forceinline__ void oct_dists(const float* const (&xr)[R], const flo
 #pragma unroll
   for (int j = 0; j < R; ++j)
 #pragma unroll
-    for (int b = 0; b < QB; ++b) {
-      const float v0 = acc[j][b][0] + dpp_f<kDppHalfMirror>(acc[j][b][0]);
-      const float v1 = acc[j][b][1] + dpp_f<kDppHalfMirror>(acc[j][b][1]);
-      const float w0 = v0 + dpp_f<kDppXor2>(v0);
-      const float w1 = v1 + dpp_f<kDppXor2>(v1);
-      const float p = w0 + w1;
-      const float r = p + dpp_f<kDppXor1>(p);
-      dist[j][b] = L2 ? r : 1.0f - r;
-    }
+    for (int b = 0; b < QB; ++b) dist[j][b] = oct_reduce<L2>(acc[j][b][0], acc[j][b][1]);
 }
+
+// LDS-DMA and counted waits, shared by the ring-buffered kernels.
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef __attribute__((address_space(1))) void* glb_vptr;
+
+// 16 bytes per lane from global straight into LDS at lds_base + 16 * lane (global_load_lds_dwordx4);
+// lds_base must be the same for the whole wave.
+__device__ __forceinline__ void load_lds16(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 16, 0, 0);
+}
+__device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 4, 0, 0);
+}
+// s_waitcnt vmcnt(N), N < 64 (gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt
+// left at their maxima so only vector-memory operations are waited for)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier of the ring-buffered kernels: waits for this wave's LDS reads and writes
+// (lgkmcnt(0)) but not for its loads in flight. Each caller first waits, with a counted vmcnt,
+// for the ring stages it is about to read. (__syncthreads() would also wait for every LDS-DMA
+// load in flight, vmcnt(0), draining the prefetch ring at every stage.) The memory clobber
+// keeps the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void ring_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // vmcnt 63 (no wait), expcnt 7, lgkmcnt 0
+  asm volatile("s_barrier" ::: "memory");
+}
+
+template <int I>
+struct Slot {
+  static constexpr int value = I;
+};
 
 // Merge of the block's per-wave sorted k-lists into one sorted k-list per query, by ranking:
 // a real key's rank in the union is its slot in its own list plus, for every other list, the

@@ -117,6 +117,195 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
   }
 }
 
+// ---- block-shared rows: 4 waves x 8 queries read every row once ---------------------------
+// The kernel above gives each wave its own item, so a group's rows are read from memory once
+// per QB queries. Here a work item belongs to a whole block: (group, chunk of rows, up to 32 of
+// the group's queries), and every row read serves all 32.
+//
+// Rows and query dims stream through a 3-slot LDS ring filled by LDS-DMA (global_load_lds)
+// two stages ahead. A stage is a window of 4 16-dim steps: the 64 rows of a row block, laid
+// out [step][row][16 floats] (16 KB), and the item's 32 queries, [step][query][16 floats]
+// (8 KB); wave w loads step w of the window for both. Wave w computes its queries 8w .. 8w+7
+// against all 64 rows in the eight-lanes-per-row order (lane t of row group g owns dims
+// 16s + o(t), +1 of row j * 8 + g; oct_reduce), accumulating over the windows of the row block;
+// after its last window the pair sums are reduced and offered to the wave's k-lists.
+//
+// Row ids travel in registers: lane L holds the id of row L of a row block. Every stage's
+// fetch first issues one plain load of the next row block's ids, then its 6 DMA loads; the
+// counted wait for a stage allows only the next stage's 6 DMA loads in flight, so it also
+// covers that stage's id load, issued before them (and stays exact where the compiler drops
+// the id load of an unfiltered item). The fetch that starts a row block moves the loaded ids
+// into `cur` (the previous ones into `prev`, still needed by the offers of the row block being
+// finished, which trail the fetches by two stages). Lanes get other rows' ids with __shfl.
+constexpr int kBlkWaves = 4, kBlkQ = 8, kBlkR = 8, kBlkRows = 8 * kBlkR, kBlkWin = 4;
+constexpr int kBlkItemQ = kBlkWaves * kBlkQ;                 // queries per item
+constexpr int kBlkRowBytes = kBlkWin * kBlkRows * 64;        // 16 KB
+constexpr int kBlkQBytes = kBlkWin * kBlkItemQ * 64;         // 8 KB
+constexpr int kBlkStage = kBlkRowBytes + kBlkQBytes;
+constexpr int kBlkLoads = kBlkStage / 1024 / kBlkWaves;      // DMA loads per wave per stage
+static_assert(kBlkRowBytes / 1024 == 4 * kBlkWaves && kBlkQBytes / 1024 == 2 * kBlkWaves, "stage split");
+
+template <int D16, int KP, bool L2>
+__global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
+    const float* __restrict__ base, uint32_t dp, const float* __restrict__ queries,
+    const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, uint32_t n_items,
+    const uint32_t* __restrict__ qperm, const uint32_t* __restrict__ allow_ids,
+    const uint64_t* __restrict__ slot_off, uint32_t L, uint64_t* __restrict__ part,
+    uint32_t* __restrict__ counter) {
+  static_assert(D16 % kBlkWin == 0 && D16 / kBlkWin >= 2, "whole windows, at least two per row block");
+  constexpr uint32_t NWIN = D16 / kBlkWin;
+  constexpr uint32_t D = D16 * 16;  // dims; queries have stride D, rows stride dp
+  __shared__ __attribute__((aligned(16))) uint8_t slot0[kBlkStage];
+  __shared__ __attribute__((aligned(16))) uint8_t slot1[kBlkStage];
+  __shared__ __attribute__((aligned(16))) uint8_t slot2[kBlkStage];
+  __shared__ uint32_t item_s;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, t = lane & 7, g = lane >> 3;
+  const int o = t < 4 ? 2 * t : 22 - 2 * t;
+
+  for (;;) {
+    __syncthreads();  // the previous item's LDS is no longer read
+    if (tid == 0) item_s = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t it = item_s;
+    if (it >= n_items) break;
+    const ListItem w = items[it];
+    const uint32_t nrows = w.le - w.lb, nrb = (nrows + kBlkRows - 1) / kBlkRows, nst = nrb * NWIN;
+    if (nrows == 0) continue;  // (the planner makes no empty items)
+    const uint32_t qw0 = uint32_t(wave) * kBlkQ;
+    const bool active = qw0 < w.nq;  // wave-uniform: a wave without queries only loads
+
+    // query rows of this lane's DMA pieces (query 16 m + lane / 4 of the item, m = 0, 1)
+    const float* qsrc[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const uint32_t qi = 16 * m + (lane >> 2);
+      qsrc[m] = queries + size_t(qperm[w.qoff + (qi < w.nq ? qi : 0)]) * D + (lane & 3) * 4;
+    }
+    WaveList<KP> lst[kBlkQ];
+    uint64_t thr[kBlkQ];
+    int kq[kBlkQ];
+#pragma unroll
+    for (int b = 0; b < kBlkQ; ++b) {
+      lst[b].init();
+      const uint32_t qi = qw0 + b;
+      kq[b] = qi < w.nq ? int(min(ks[qperm[w.qoff + qi]], uint32_t(KP))) : 0;
+      thr[b] = kq[b] > 0 ? kPadKey : 0;  // k == 0 admits nothing
+    }
+    // ids of row block rb, lane L: row L (clamped into the chunk; a direct group's rows are lb + i)
+    auto id_load = [&](uint32_t rb) -> uint32_t {
+      const uint32_t i = rb * kBlkRows + uint32_t(lane), ic = i < nrows ? i : 0u;
+      return w.direct ? allow_ids[0] : allow_ids[w.lb + ic];
+    };
+    auto id_of = [&](uint32_t v, uint32_t rb, uint32_t r) -> uint32_t {  // row r of row block rb
+      const uint32_t i = rb * kBlkRows + r;
+      return w.direct ? w.lb + (i < nrows ? i : 0u) : uint32_t(__shfl(int(v), int(r), 64));
+    };
+    uint32_t id_next = id_load(0), id_cur = 0, id_prev = 0;
+    wait_vm<0>();  // qperm / ks / the first ids (and the previous item's stores)
+
+    auto fetch = [&](uint8_t* dst, uint32_t st) __attribute__((always_inline)) {
+      const uint32_t rb = st / NWIN, win = st - rb * NWIN;
+      if (win == 0) {
+        id_prev = id_cur;
+        id_cur = id_next;
+      }
+      uint32_t row[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) row[m] = id_of(id_cur, rb, 16 * m + (lane >> 2));
+      id_next = id_load(rb + 1 < nrb ? rb + 1 : rb);
+      const uint32_t s = win * kBlkWin + uint32_t(wave);  // the step this wave loads
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        load_lds16(base + size_t(row[m]) * dp + s * 16 + (lane & 3) * 4, dst + wave * 4096 + m * 1024);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) load_lds16(qsrc[m] + s * 16, dst + kBlkRowBytes + (2 * wave + m) * 1024);
+    };
+
+    float acc[kBlkR][kBlkQ][2];
+    auto compute = [&](const uint8_t* src, uint32_t st) __attribute__((always_inline)) {
+      const uint32_t rb = st / NWIN, win = st - rb * NWIN;
+      if (win == 0) {
+#pragma unroll
+        for (int j = 0; j < kBlkR; ++j)
+#pragma unroll
+          for (int b = 0; b < kBlkQ; ++b) acc[j][b][0] = acc[j][b][1] = 0.f;
+      }
+      const float* xs = reinterpret_cast<const float*>(src);
+      const float* qs = reinterpret_cast<const float*>(src + kBlkRowBytes) + qw0 * 16;
+#pragma unroll
+      for (int sw = 0; sw < kBlkWin; ++sw) {
+        float2 xv[kBlkR];
+#pragma unroll
+        for (int j = 0; j < kBlkR; ++j)
+          xv[j] = *reinterpret_cast<const float2*>(xs + (sw * kBlkRows + j * 8 + g) * 16 + o);
+#pragma unroll
+        for (int b = 0; b < kBlkQ; ++b) {
+          const float2 qv = *reinterpret_cast<const float2*>(qs + (sw * kBlkItemQ + b) * 16 + o);
+#pragma unroll
+          for (int j = 0; j < kBlkR; ++j) {
+            acc[j][b][0] = fma_term<L2>(qv.x, xv[j].x, acc[j][b][0]);
+            acc[j][b][1] = fma_term<L2>(qv.y, xv[j].y, acc[j][b][1]);
+          }
+        }
+      }
+      if (win + 1 == NWIN) {  // the row block is complete: distances to the k-lists
+#pragma unroll
+        for (int j = 0; j < kBlkR; ++j) {
+          const uint32_t r = uint32_t(j * 8 + g), i = rb * kBlkRows + r;
+          const bool ok = i < nrows && t == 0;
+          // the fetches moved on to row block rb + 1 two stages ago, if there is one
+          const uint32_t row = id_of(rb + 1 < nrb ? id_prev : id_cur, rb, r);
+#pragma unroll
+          for (int b = 0; b < kBlkQ; ++b) {
+            const float dist = oct_reduce<L2>(acc[j][b][0], acc[j][b][1]);
+            if (kq[b] == 0) continue;
+            thr[b] = offer<KP>(lst[b], thr[b], ok ? make_key(dist, row) : kPadKey, lane, kq[b]);
+          }
+        }
+      }
+    };
+    auto slot = [&](auto I) -> uint8_t* {
+      constexpr int v = decltype(I)::value % 3;
+      return v == 0 ? slot0 : v == 1 ? slot1 : slot2;
+    };
+    // Consume stage st from slot I; refill slot I + 2 (consumed at st - 1) with stage st + 2.
+    // Every step issues the same loads (past the last stage it fetches the last stage again,
+    // into a slot nobody reads; the next item's first barrier drains them), so the counted
+    // wait is the same everywhere and the compiler's own load tracking stays exact.
+    auto step = [&](auto I, uint32_t st) __attribute__((always_inline)) {
+      wait_vm<kBlkLoads>();  // stage st + 1's DMA loads may stay in flight
+      ring_barrier();
+      fetch(slot(Slot<decltype(I)::value + 2>()), min(st + 2, nst - 1));
+      if (active) compute(slot(I), st);
+    };
+    fetch(slot0, 0);  // nst >= NWIN >= 2: an item has at least one row
+    fetch(slot1, 1);
+    for (uint32_t st = 0;;) {  // leaves the loop right after the last stage
+      step(Slot<0>(), st);
+      if (++st == nst) break;
+      step(Slot<1>(), st);
+      if (++st == nst) break;
+      step(Slot<2>(), st);
+      if (++st == nst) break;
+    }
+
+    wait_vm<0>();  // the refetches past the last stage land before the slots are reused
+    if (active) {
+#pragma unroll
+      for (int b = 0; b < kBlkQ; ++b) {
+        const uint32_t qi = qw0 + b;
+        if (qi >= w.nq) continue;
+        uint64_t* dst = part + (slot_off[qperm[w.qoff + qi]] + w.chunk) * L;
+#pragma unroll
+        for (int r = 0; r < KP / 64; ++r) {
+          const uint32_t sl = r * 64 + lane;
+          if (sl < L) dst[sl] = lst[b].e[r];
+        }
+      }
+    }
+  }
+}
+
 // ---- host side ---------------------------------------------------------------------------
 
 namespace {
@@ -205,7 +394,28 @@ hipError_t launch_lists_oct(bool l2, int kp, const ListsArgs& a) {
   }
 }
 
+template <int D16, int KP, bool L2>
+hipError_t launch_block_t(const ListsArgs& a) {
+  hipLaunchKernelGGL((lists_block_kernel<D16, KP, L2>), a.grid, dim3(64 * kBlkWaves), 0, a.st, a.base, a.dp, a.q,
+                     a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off, a.L, a.part, a.counter);
+  return hipGetLastError();
+}
+
+template <int D16>
+hipError_t launch_block_k(bool l2, int kp, const ListsArgs& a) {
+  if (kp == 64) return l2 ? launch_block_t<D16, 64, true>(a) : launch_block_t<D16, 64, false>(a);
+  return l2 ? launch_block_t<D16, 128, true>(a) : launch_block_t<D16, 128, false>(a);
+}
+
 hipError_t launch_lists_tile(int qb, int r, int lpr, bool l2, int kp, const ListsArgs& a) {
+  if (qb == kBlkItemQ) {  // block-shared rows (r = kBlkR, eight lanes per row)
+    switch (a.d) {
+      case 384: return launch_block_k<24>(l2, kp, a);
+      case 768: return launch_block_k<48>(l2, kp, a);
+      case 128: return launch_block_k<8>(l2, kp, a);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (lpr == 8) {
     if (qb == 8 && r == 4) return launch_lists_oct<8, 4>(l2, kp, a);
     if (qb == 8 && r == 2) return launch_lists_oct<8, 2>(l2, kp, a);
@@ -230,6 +440,7 @@ int hip_status(hipError_t e) {
 }  // namespace
 
 bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
+  if (qb == kBlkItemQ) return r == kBlkR && lpr == 8 && (dim == 128 || dim == 384 || dim == 768);
   if (lpr == 8)
     return (dim == 128 || dim == 384 || dim == 768) && ((qb == 8 && (r == 2 || r == 4)) || (qb == 16 && r == 2));
   return (qb == 8 && r == 2) || (qb == 4 && r == 4) || (qb == 2 && r == 4) || (qb == 1 && r == 4);
@@ -319,13 +530,16 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   // queries per item: 4 waves x qb x dp floats of LDS must fit in 64 KB (160 KB for a
   // requested tile)
   int qb = tiles.qb ? tiles.qb : 4, r = tiles.r;
+  const bool block = qb == kBlkItemQ;  // block-shared rows: its ring is static LDS
   const size_t lds_cap = tiles.qb ? size_t(160) << 10 : size_t(65536);
-  while (qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) qb /= 2;
-  if (size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) {
+  if (block && !lists_tile_supported(qb, kBlkR, 8, dim)) qb = 8;
+  while (!block && qb > 1 && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) qb /= 2;
+  if (!block && size_t(kWavesPerBlock) * qb * dp * sizeof(float) > lds_cap) {
     err = "dim too large for the filtered scan (at most 4096)";
     return VSG_ERR_INVALID_ARGUMENT;
   }
-  int lpr = tiles.lanes_per_row == 8 ? 8 : 4;
+  int lpr = tiles.lanes_per_row == 8 || block ? 8 : 4;
+  if (block) r = kBlkR;
   if (!r) r = lpr == 8 ? (qb >= 16 ? 2 : 4) : (qb >= 8 ? 2 : 4);
   if (!lists_tile_supported(qb, r, lpr, dim)) {  // fall back to the nearest quad tile
     lpr = 4;
@@ -339,7 +553,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     const auto sp = span_of(kv.first);
     visits += (sp.second - sp.first) * ((kv.second.size() + qb - 1) / qb);
   }
-  const uint64_t wave_slots = uint64_t(num_cus) * 16;
+  const uint64_t wave_slots = uint64_t(num_cus) * (block ? 2 : 16);  // item slots
   uint64_t C = visits / (wave_slots * 2) + 1;
   C = std::min<uint64_t>(8192, std::max<uint64_t>(256, (C + 63) / 64 * 64));
   if (tiles.chunk) C = std::max<uint64_t>(64, (tiles.chunk + 63) / 64 * 64);
@@ -355,7 +569,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     used->r = r;
     used->lanes_per_row = lpr;
     used->chunk = C;
-    used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : 4;
+    used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
   }
   std::vector<ListItem> items;
   std::vector<uint32_t> qperm;
@@ -421,10 +635,10 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   float* d_dists = reinterpret_cast<float*>(d_ids + nq * kmax);
   if (!items.empty()) {
     ListsArgs a;
-    const uint64_t want_blocks = (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int bpc = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : 4;
+    const uint64_t want_blocks = block ? items.size() : (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int bpc = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
     a.grid = dim3(unsigned(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * bpc)));
-    a.lds = size_t(kWavesPerBlock) * qb * dp * sizeof(float);
+    a.lds = block ? 0 : size_t(kWavesPerBlock) * qb * dp * sizeof(float);
     a.st = st;
     a.base = base;
     a.dp = uint32_t(dp);

@@ -233,30 +233,6 @@ __device__ __forceinline__ float row16_min(float v) {
   return v;
 }
 
-typedef __attribute__((address_space(3))) void* lds_vptr;
-typedef __attribute__((address_space(1))) void* glb_vptr;
-
-// 16 bytes per lane from global straight into LDS at lds_base + 16 * lane (global_load_lds_dwordx4);
-// lds_base must be the same for the whole wave.
-__device__ __forceinline__ void load_lds16(const void* g, void* lds_base) {
-  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 16, 0, 0);
-}
-__device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
-  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 4, 0, 0);
-}
-// s_waitcnt vmcnt(N), N < 64 (gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt
-// left at their maxima so only vector-memory operations are waited for)
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-template <int I>
-struct Slot {
-  static constexpr int value = I;
-};
-
 template <int PREC, int KS>
 struct StageShape {
   static constexpr int RF = rows_frags(PREC, KS);
@@ -353,15 +329,15 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     }
   };
 
-  // consume stage st from slot I; refill slot I + 3 (consumed at st - 1) with stage st + 3
+  // Consume stage st from slot I; refill slot I + 3 (consumed at st - 1) with stage st + 3.
+  // Every step issues the same loads (past the last stage it fetches the last stage again, into
+  // a slot nobody reads), so the counted wait is the same everywhere and the compiler's own load
+  // tracking stays exact.
   auto step = [&](auto I, uint32_t st) {
-    const uint32_t later = min(nstages - 1 - st, uint32_t(kAhead - 1));
-    if (later >= 2) wait_vm<2 * LOADS>();
-    else if (later == 1) wait_vm<LOADS>();
-    else wait_vm<0>();
-    __syncthreads();  // every wave's part of stage st is in LDS; stage st - 1 is consumed
+    wait_vm<(kAhead - 1) * LOADS>();  // the two later stages may stay in flight
+    ring_barrier();  // every wave's part of stage st is in LDS; stage st - 1 is consumed
     if (PASS == 1 && st > 0) flush_min(st - 1);
-    if (st + kAhead < nstages) fetch(slot(Slot<decltype(I)::value + kAhead>()), st + kAhead);
+    fetch(slot(Slot<decltype(I)::value + kAhead>()), min(st + kAhead, nstages - 1));
     const uint8_t* stage = slot(I);
     const FT* as = reinterpret_cast<const FT*>(stage);
     const float* aux = reinterpret_cast<const float*>(stage + SH::STAGE_Q);
@@ -433,16 +409,22 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     }
   };
 
-  if (nstages > 0) fetch(slot0, 0);
-  if (nstages > 1) fetch(slot1, 1);
-  if (nstages > 2) fetch(slot2, 2);
-  for (uint32_t st0 = 0; st0 < nstages; st0 += 4) {
-    step(Slot<0>(), st0);
-    if (st0 + 1 < nstages) step(Slot<1>(), st0 + 1);
-    if (st0 + 2 < nstages) step(Slot<2>(), st0 + 2);
-    if (st0 + 3 < nstages) step(Slot<3>(), st0 + 3);
+  // nstages >= 1 (the launch has at least one query)
+  fetch(slot0, 0);
+  fetch(slot1, min(1u, nstages - 1));
+  fetch(slot2, min(2u, nstages - 1));
+  for (uint32_t st = 0;;) {  // leaves the loop right after the last stage
+    step(Slot<0>(), st);
+    if (++st == nstages) break;
+    step(Slot<1>(), st);
+    if (++st == nstages) break;
+    step(Slot<2>(), st);
+    if (++st == nstages) break;
+    step(Slot<3>(), st);
+    if (++st == nstages) break;
   }
-  if (PASS == 1 && nstages > 0) {  // tile minima of the last stage
+  wait_vm<0>();  // the refetches past the last stage land before the block's LDS is released
+  if (PASS == 1) {  // tile minima of the last stage
     __syncthreads();
     flush_min(nstages - 1);
   }
@@ -905,6 +887,7 @@ int short_ks(uint32_t d) {
 int short_tile_rows_for(int prec, int ks) { return kWaves * rows_frags(prec, ks) * 16; }
 
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a) {
+  if (a.nq == 0 || a.n_blocks == 0) return hipErrorInvalidValue;  // a block streams at least one stage
   return prec == 1 ? launch_mfma_p<1>(pass, a) : launch_mfma_p<2>(pass, a);
 }
 
