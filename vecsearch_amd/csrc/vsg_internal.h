@@ -83,8 +83,9 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {
-  void* dev[8] = {};      // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out)
-  size_t dev_cap[8] = {};
+  void* dev[9] = {};      // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out,
+                          // per-query thresholds)
+  size_t dev_cap[9] = {};
   void* host[3] = {};     // pinned staging (queries, ids out, dists out)
   size_t host_cap[3] = {};
   uint32_t* counter = nullptr;

@@ -345,7 +345,11 @@ __device__ __forceinline__ uint64_t offer(WaveList<KP>& lst, uint64_t thr, uint6
     const int src = __builtin_ctzll(m);
     m &= m - 1;
     const uint64_t ck = readlane64(key, src);
-    if (ck < thr) thr = lst.insert(ck, lane, k);
+    if (ck < thr) {
+      // the list's k-th key, or the (tighter) threshold the scan started from
+      const uint64_t t = lst.insert(ck, lane, k);
+      thr = t < thr ? t : thr;
+    }
   }
   return thr;
 }

@@ -12,6 +12,12 @@
 // queries). Waves pull items from a global counter until none is left, keep a k-list per
 // query, and write it to the query's slot for that chunk; the merge kernel then reduces the
 // chunk lists of each query.
+//
+// Chunks of one query share an admission threshold: qthr[q] (a key, kPadKey at the start) is
+// lowered with atomicMin to the k-th key of every complete chunk list, and a chunk's list
+// starts from it instead of from empty. A key left out is then above k keys of some other
+// chunk, so the merged top-k is unchanged, and later chunks skip most of the list inserts
+// that filling k slots from empty costs (k (1 + ln(C / k)) per chunk for k = 100).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
     const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, uint32_t n_items,
     const uint32_t* __restrict__ qperm, const uint32_t* __restrict__ allow_ids,
     const uint64_t* __restrict__ slot_off, uint32_t L, uint64_t* __restrict__ part,
-    uint32_t* __restrict__ counter) {
+    uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
   extern __shared__ float lds[];
   constexpr int kRowsPerStep = 64 / LPR;  // rows per wave instruction
   const int tid = threadIdx.x, lane = tid & 63, t = lane & (LPR - 1), g = lane / LPR;
@@ -73,8 +79,10 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
 #pragma unroll
     for (int b = 0; b < QB; ++b) {
       lst[b].init();
-      kq[b] = uint32_t(b) < w.nq ? int(min(ks[qperm[w.qoff + b]], uint32_t(KP))) : 0;
-      thr[b] = kq[b] > 0 ? kPadKey : 0;  // k == 0 admits nothing
+      const uint32_t q = uint32_t(b) < w.nq ? qperm[w.qoff + b] : 0u;
+      kq[b] = uint32_t(b) < w.nq ? int(min(ks[q], uint32_t(KP))) : 0;
+      // k == 0 admits nothing; else the tightest bound other chunks of the query have published
+      thr[b] = kq[b] > 0 ? __hip_atomic_load(qthr + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     }
 
     for (uint32_t r0 = w.lb; r0 < w.le; r0 += kRowsPerStep * R) {
@@ -107,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
     for (int b = 0; b < QB; ++b) {
       if (uint32_t(b) >= w.nq) continue;
       const uint32_t q = qperm[w.qoff + b];
+      if (lane == 0 && thr[b] != kPadKey && kq[b] > 0) atomicMin(reinterpret_cast<unsigned long long*>(qthr + q), thr[b]);
       uint64_t* dst = part + (slot_off[q] + w.chunk) * L;
 #pragma unroll
       for (int r = 0; r < KP / 64; ++r) {
@@ -151,7 +160,7 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
     const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, uint32_t n_items,
     const uint32_t* __restrict__ qperm, const uint32_t* __restrict__ allow_ids,
     const uint64_t* __restrict__ slot_off, uint32_t L, uint64_t* __restrict__ part,
-    uint32_t* __restrict__ counter) {
+    uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
   static_assert(D16 % kBlkWin == 0 && D16 / kBlkWin >= 2, "whole windows, at least two per row block");
   constexpr uint32_t NWIN = D16 / kBlkWin;
   constexpr uint32_t D = D16 * 16;  // dims; queries have stride D, rows stride dp
@@ -188,8 +197,9 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
     for (int b = 0; b < kBlkQ; ++b) {
       lst[b].init();
       const uint32_t qi = qw0 + b;
-      kq[b] = qi < w.nq ? int(min(ks[qperm[w.qoff + qi]], uint32_t(KP))) : 0;
-      thr[b] = kq[b] > 0 ? kPadKey : 0;  // k == 0 admits nothing
+      const uint32_t q = qi < w.nq ? qperm[w.qoff + qi] : 0u;
+      kq[b] = qi < w.nq ? int(min(ks[q], uint32_t(KP))) : 0;
+      thr[b] = kq[b] > 0 ? __hip_atomic_load(qthr + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     }
     // ids of row block rb, lane L: row L (clamped into the chunk; a direct group's rows are lb + i)
     auto id_load = [&](uint32_t rb) -> uint32_t {
@@ -295,7 +305,9 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
       for (int b = 0; b < kBlkQ; ++b) {
         const uint32_t qi = qw0 + b;
         if (qi >= w.nq) continue;
-        uint64_t* dst = part + (slot_off[qperm[w.qoff + qi]] + w.chunk) * L;
+        const uint32_t q = qperm[w.qoff + qi];
+        if (lane == 0 && thr[b] != kPadKey && kq[b] > 0) atomicMin(reinterpret_cast<unsigned long long*>(qthr + q), thr[b]);
+        uint64_t* dst = part + (slot_off[q] + w.chunk) * L;
 #pragma unroll
         for (int r = 0; r < KP / 64; ++r) {
           const uint32_t sl = r * 64 + lane;
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
 
 namespace {
 
-enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut };
+enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut, kDThr };
 
 hipError_t dev_reserve(ListsWorkspace& ws, int i, size_t bytes) {
   if (bytes <= ws.dev_cap[i]) return hipSuccess;
@@ -350,6 +362,7 @@ struct ListsArgs {
   uint32_t L;
   uint64_t* part;
   uint32_t* counter;
+  uint64_t* qthr;
 };
 
 template <int QB, int R, int D16, int KP, bool L2, int LPR>
@@ -361,7 +374,7 @@ hipError_t launch_lists_t(const ListsArgs& a) {
   }
   hipLaunchKernelGGL((lists_scan_kernel<QB, R, D16, KP, L2, LPR>), a.grid, dim3(kBlock), a.lds, a.st,
                      a.base, a.dp, a.d, a.q, a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off,
-                     a.L, a.part, a.counter);
+                     a.L, a.part, a.counter, a.qthr);
   return hipGetLastError();
 }
 
@@ -397,7 +410,7 @@ hipError_t launch_lists_oct(bool l2, int kp, const ListsArgs& a) {
 template <int D16, int KP, bool L2>
 hipError_t launch_block_t(const ListsArgs& a) {
   hipLaunchKernelGGL((lists_block_kernel<D16, KP, L2>), a.grid, dim3(64 * kBlkWaves), 0, a.st, a.base, a.dp, a.q,
-                     a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off, a.L, a.part, a.counter);
+                     a.ks, a.items, a.n_items, a.qperm, a.ids, a.slot_off, a.L, a.part, a.counter, a.qthr);
   return hipGetLastError();
 }
 
@@ -447,7 +460,7 @@ bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
 }
 
 void ListsWorkspace::release() {
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < 9; ++i) {
     if (dev[i]) (void)hipFree(dev[i]);
     dev[i] = nullptr;
     dev_cap[i] = 0;
@@ -613,6 +626,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   LISTS_HIP(dev_reserve(ws, kDIds, std::max<uint64_t>(1, total) * sizeof(uint32_t)));
   LISTS_HIP(dev_reserve(ws, kDPart, std::max<uint64_t>(1, n_slots) * L * sizeof(uint64_t)));
   LISTS_HIP(dev_reserve(ws, kDOut, nq * kmax * (sizeof(uint32_t) + sizeof(float))));
+  LISTS_HIP(dev_reserve(ws, kDThr, nq * sizeof(uint64_t)));
   LISTS_HIP(host_reserve(ws, 0, qbytes));
   LISTS_HIP(host_reserve(ws, 1, nq * kmax * sizeof(uint32_t)));
   LISTS_HIP(host_reserve(ws, 2, nq * kmax * sizeof(float)));
@@ -630,6 +644,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   if (total)
     LISTS_HIP(hipMemcpyAsync(ws.dev[kDIds], allow_ids, total * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   LISTS_HIP(hipMemsetAsync(ws.counter, 0, sizeof(uint32_t), st));
+  LISTS_HIP(hipMemsetAsync(ws.dev[kDThr], 0xFF, nq * sizeof(uint64_t), st));  // kPadKey
 
   uint32_t* d_ids = static_cast<uint32_t*>(ws.dev[kDOut]);
   float* d_dists = reinterpret_cast<float*>(d_ids + nq * kmax);
@@ -653,6 +668,7 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     a.L = L;
     a.part = static_cast<uint64_t*>(ws.dev[kDPart]);
     a.counter = ws.counter;
+    a.qthr = static_cast<uint64_t*>(ws.dev[kDThr]);
     const bool l2 = metric == VSG_METRIC_L2;
     const int kp = kmax <= 64 ? 64 : 128;
     if (timing) timing->begin(st);
