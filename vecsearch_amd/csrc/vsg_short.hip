@@ -686,6 +686,51 @@ __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
     }
 }
 
+// The kk-th smallest (1-based, 1 <= kk <= c) of the high words of buf[0, c) (the order-preserving
+// bits of the estimates), by a 256-thread block: radix select, 8 bits a pass; wave 0 finds
+// each pass's bin with a prefix sum over the 256 bins (4 a lane). Uses hist (256) and sel (2).
+__device__ uint32_t block_select_hi32(const uint64_t* buf, uint32_t c, uint32_t kk, uint32_t* hist, uint32_t* sel,
+                                      uint32_t tid) {
+  uint32_t prefix = 0, pmask = 0, rank = kk;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < c; i += 256) {
+      const uint32_t x = uint32_t(buf[i] >> 32);
+      if ((x & pmask) == prefix) atomicAdd(&hist[(x >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t h[4], sum = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sum += (h[j] = hist[4 * tid + j]);
+      uint32_t incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (tid >= uint32_t(off)) incl += y;
+      }
+      uint32_t acc = incl - sum;
+      if (acc < rank && rank <= incl) {  // exactly one lane
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (acc + h[j] >= rank) {
+            sel[0] = 4 * tid + j;
+            sel[1] = rank - acc;
+            break;
+          }
+          acc += h[j];
+        }
+      }
+    }
+    __syncthreads();
+    prefix |= sel[0] << shift;
+    pmask |= 255u << shift;
+    rank = sel[1];
+  }
+  return prefix;
+}
+
 // Handles the queries with cmin <= count <= CAP; with final_pass set, flags those above CAP.
 template <int D16, bool L2, int CAP>
 __global__ __launch_bounds__(256) void rescore_kernel(
@@ -699,6 +744,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   uint64_t* buf = dyn;                                // CAP keys
   float* q_lds = reinterpret_cast<float*>(dyn + CAP);  // dp floats
   __shared__ uint32_t nsel;
+  __shared__ uint32_t hist[256], sel[2];
   const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
   const uint32_t kq = min(ks ? ks[q] : k, ldo);
   uint32_t* oi = out_ids + size_t(q) * ldo;
@@ -719,23 +765,23 @@ __global__ __launch_bounds__(256) void rescore_kernel(
     od[i] = __builtin_inff();
   }
   for (uint32_t i = tid; i < dp; i += 256) q_lds[i] = i < d ? queries[size_t(q) * d + i] : 0.f;
-  // candidates sorted by (dh, id)
-  uint32_t n2 = 256;
-  while (n2 < c) n2 <<= 1;
-  for (uint32_t i = tid; i < n2; i += 256) buf[i] = i < c ? cand[size_t(q) * kCandCap + i] : kPadKey;
-  __syncthreads();
-  bitonic_sort_lds(buf, n2, tid);
-  // keep dh <= Dk + 2E
-  const uint32_t kk = min(kq, c);
+  // Dk = the kk-th smallest estimate; keep dh <= Dk + 2E (compacted, in no particular order)
+  const uint64_t* cq = cand + size_t(q) * kCandCap;
+  for (uint32_t i = tid; i < c; i += 256) buf[i] = cq[i];
   if (tid == 0) nsel = 0;
   __syncthreads();
+  const uint32_t kk = min(kq, c);
   if (kk > 0) {
-    const float lim = round_up_f(double(key_dist(buf[kk - 1])) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
-    for (uint32_t i = tid; i < c; i += 256)
-      if (key_dist(buf[i]) <= lim) atomicMax(&nsel, i + 1);
+    const float Dk = unord_bits(block_select_hi32(buf, c, kk, hist, sel, tid));
+    const float lim = round_up_f(double(Dk) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    __syncthreads();  // every read of buf by the select is done
+    for (uint32_t i = tid; i < c; i += 256) {
+      const uint64_t v = cq[i];
+      if (key_dist(v) <= lim) buf[atomicAdd(&nsel, 1u)] = v;
+    }
   }
   __syncthreads();
-  const uint32_t r = nsel;  // sorted, so the kept candidates are a prefix
+  const uint32_t r = nsel;
   if (tid == 0) rescored[q] = r;
   // exact distances, 16 rows per wave instruction group, 4 waves; a quad reads its
   // candidate's row id and later overwrites that slot with the exact key
