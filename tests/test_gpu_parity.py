@@ -281,6 +281,32 @@ def test_shortlist_matches_oracle(mixture_70k, method, k):
     assert_same(got, exp)
 
 
+@pytest.mark.parametrize("method", ["bf16", "fp8"])
+@pytest.mark.parametrize("nq", [1, 16, 100, 128])
+@pytest.mark.parametrize("k", [10, 100])
+def test_shortlist_small_batches(mixture_70k, method, nq, k):
+    # up to 128 queries (d <= 384) take the queries-stationary pass 2: same bits as the oracle,
+    # and as the same queries inside a larger batch (rows-stationary pass 2)
+    x, q, gpu = mixture_70k
+    got = gpu.search(q[:nq], k, method=method)
+    assert_same(got, oracle.search(x, q[:nq], k, "ip", threads=8))
+    big = gpu.search(q, k, method=method)
+    np.testing.assert_array_equal(got[0], big[0][:nq])
+    np.testing.assert_array_equal(bits(got[1]), bits(big[1][:nq]))
+
+
+@pytest.mark.parametrize("n,d,nq", [(5003, 128, 37), (1000, 64, 5), (70_001, 768, 64), (70_001, 768, 65)])
+def test_shortlist_small_batch_shapes(n, d, nq):
+    # ragged row counts (a partial last fragment, blocks of a few rows) and d = 768, where 64
+    # queries are one query group and 65 are not
+    x = dg.mixture(n, d, seed=90)
+    q = dg.mixture(nq, d, seed=91)
+    gpu = vs.GpuBruteForceIndex(d, "cosine")
+    gpu.add(x)
+    for method in ("bf16", "fp8"):
+        assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
+
+
 @pytest.mark.parametrize("d", [100, 128, 256, 768])
 def test_shortlist_other_dims(d):
     x = dg.mixture(66_000, d, seed=42)

@@ -473,14 +473,22 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   if ((e = w.s_rescored.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_nover.reserve(2)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
-  // pass-2 emission regions, one per wave: a region of R rows receives about
-  // round * (candidates per query) * R / n entries (rows are in no particular order); sized for
-  // 4x that at 2,048 candidates per query and at least 1,024, the most a region can receive
-  // being R * round (an overflow sends the round to the exact scan)
-  const size_t region_rows = tile_rows / size_t(vsg::short_waves());
-  const size_t expect = round * 2048 * region_rows / std::max<size_t>(1, h->n);
-  const uint32_t wave_cap = uint32_t(std::min(region_rows * round, std::max<size_t>(1024, 4 * expect)));
-  const size_t n_regions = size_t(n_tiles) * vsg::short_waves();
+  // Pass 2 runs queries-stationary when the batch fits one block's registers (one round),
+  // else rows-stationary over 512-row tiles.
+  uint32_t qs_blocks = 0, qs_groups = 0, qs_rows = 0, qs_qpr = 0;
+  const bool qs = vsg::short_qs_plan(p, ks, uint32_t(h->n), uint32_t(nq), h->num_cus, &qs_blocks, &qs_groups,
+                                     &qs_rows, &qs_qpr);
+  // pass-2 emission regions, one per wave: a region of R rows and Q queries receives about
+  // Q * (candidates per query) * R / n entries (rows are in no particular order); sized for
+  // 4x that at 2,048 candidates per query and at least 1,024 (4,096 for the few large regions
+  // of the queries-stationary pass), the most a region can receive being R * Q (an overflow
+  // sends the round to the exact scan)
+  const size_t region_rows = qs ? qs_rows : tile_rows / size_t(vsg::short_waves());
+  const size_t region_q = qs ? qs_qpr : round;
+  const size_t expect = region_q * 2048 * region_rows / std::max<size_t>(1, h->n);
+  const uint32_t wave_cap =
+      uint32_t(std::min(region_rows * region_q, std::max<size_t>(qs ? 4096 : 1024, 4 * expect)));
+  const size_t n_regions = qs ? size_t(qs_blocks) * qs_groups * 4 : size_t(n_tiles) * vsg::short_waves();
   if ((e = w.s_emit.reserve(n_regions * wave_cap)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_slice.reserve(round * vsg::short_scatter_slices())) != hipSuccess) return fail_hip("workspace", e);
@@ -535,7 +543,14 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
       a.tstride = 1;
       a.n_space = a.n_blocks = n_tiles;
-      VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
+      if (qs) {
+        a.qs_blocks = qs_blocks;
+        a.qs_groups = qs_groups;
+        a.qs_rows = qs_rows;
+        VSG_HIP(vsg::launch_mfma_qs(p, a));
+      } else {
+        VSG_HIP(vsg::launch_mfma_scan(p, 2, a));
+      }
       ts.launches++;
     }
     {

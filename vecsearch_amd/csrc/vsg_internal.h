@@ -51,8 +51,16 @@ struct ShortScanArgs {
   uint32_t* count;
   uint32_t* overflow;     // blocks whose region overflowed
   uint32_t* slice;        // nq * short_scatter_slices() counters of the scatter
+  // queries-stationary pass 2 (short_qs_plan): blocks per query group (0: rows-stationary
+  // pass 2), query groups, rows per block
+  uint32_t qs_blocks = 0, qs_groups = 0, qs_rows = 0;
 };
 size_t short_scatter_slices();
+// Queries-stationary pass 2 for a batch of at most one query group: its launch shape (blocks of
+// `rows` rows), or false where it does not apply.
+bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint32_t* blocks, uint32_t* groups,
+                   uint32_t* rows, uint32_t* queries_per_region);
+hipError_t launch_mfma_qs(int prec, const ShortScanArgs& a);
 int short_ks(uint32_t d);
 int short_cand_cap();
 int short_tile_rows_for(int prec, int ks);

@@ -431,10 +431,214 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   if (PASS == 2 && lane == 0) emit_count[size_t(j) * kWaves + wave] = n_emit;
 }
 
+// ---- pass 2 for small batches (queries stationary, rows streaming) ------------------------
+// When a batch's query fragments fit the registers of one block's four waves, the rows need
+// not stay in registers: wave w keeps fragments w * QW .. + QW of its query group (A operands
+// of every k-step, with their score cuts and scales), and the block streams its range of row
+// fragments through an LDS ring (NBUF slots of SF fragments, filled by LDS-DMA NBUF - 1 stages
+// ahead). Every wave runs its fragments against every row
+// fragment, so HBM reads each row once per query group. The MFMA operands and their order are
+// those of mfma_scan_kernel, so the estimates are the same bits. Emission: wave w of block b,
+// query group y appends to its own region ((y * gridDim.x + b) * 4 + w) entries
+// (ord(dh) << 32) | (query << kQsShift) | (row - first row of the block).
+constexpr int kQsWaves = 4;
+constexpr uint32_t kQsShift = 20;          // rows of a block < 2^20
+constexpr uint32_t kQsMaxRows = 4096;      // rows per block (FP8 scales of the block in LDS)
+
+// row fragments per stage: the largest of 8, 4, 2, 1 whose stage fits 12 KB and splits into
+// whole 1 KB pieces, the same number for each wave; 1 if none fits (d = 768 in BF16: 24 KB).
+// A block's four-slot ring is then at most 48 KB (the compiler tracks LDS-DMA targets for only
+// a few distinct LDS objects, so the ring has four slots, not more), and up to 3 blocks share a
+// CU (qs_bpc: as many as the LDS holds), each with up to 3 stages in flight.
+__host__ __device__ constexpr bool qs_sf_ok(int prec, int ks, int sf) {
+  return sf * ks * (prec == 1 ? 1024 : 512) <= (12 << 10) && (sf * ks * (prec == 1 ? 1024 : 512)) % (1024 * kQsWaves) == 0;
+}
+__host__ __device__ constexpr int qs_sf(int prec, int ks) {
+  return qs_sf_ok(prec, ks, 8) ? 8 : qs_sf_ok(prec, ks, 4) ? 4 : qs_sf_ok(prec, ks, 2) ? 2 : 1;
+}
+__host__ __device__ constexpr int qs_lds(int prec, int ks) {  // the ring and, for FP8, the row scales
+  return 4 * qs_sf(prec, ks) * ks * (prec == 1 ? 1024 : 512) + (prec == 2 ? int(kQsMaxRows) * 4 : 0);
+}
+__host__ __device__ constexpr int qs_bpc(int prec, int ks) {  // blocks per CU
+  return (160 << 10) / qs_lds(prec, ks) > 3 ? 3 : (160 << 10) / qs_lds(prec, ks) < 1 ? 1 : (160 << 10) / qs_lds(prec, ks);
+}
+// query fragments per wave: A operands of all k-steps in at most 96 registers
+__host__ __device__ constexpr int qs_qw(int prec, int ks) { return 2 * ks * (prec == 1 ? 4 : 2) <= 96 ? 2 : 1; }
+
+template <int PREC, int KS>
+struct QsShape {
+  static constexpr int CB = Frag<PREC>::kChunkBytes;
+  static constexpr int SF = qs_sf(PREC, KS);
+  static constexpr int STAGE = SF * KS * CB;
+  static constexpr int NBUF = 4;
+  static constexpr int LOADS = STAGE / 1024 / kQsWaves;  // DMA loads per wave per stage
+  static constexpr int QW = qs_qw(PREC, KS);
+  static_assert(STAGE % (1024 * kQsWaves) == 0 && (KS * CB) % 1024 == 0, "stage shape");
+};
+
+template <int PREC, int KS>
+__global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kernel(
+    const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
+    const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t rows_per_block,
+    uint64_t* __restrict__ emit, uint32_t wave_cap, uint32_t* __restrict__ emit_count) {
+  typedef typename Frag<PREC>::T FT;
+  typedef QsShape<PREC, KS> SH;
+  constexpr int SF = SH::SF, NBUF = SH::NBUF, QW = SH::QW, LOADS = SH::LOADS, ST = SH::STAGE;
+  constexpr int FB = KS * SH::CB;  // bytes of one row fragment (all k-steps)
+  // distinct LDS objects per slot: the compiler then waits only on the counted vmcnt
+  static_assert(NBUF == 4, "four ring slots");
+  __shared__ __attribute__((aligned(16))) uint8_t s0[ST];
+  __shared__ __attribute__((aligned(16))) uint8_t s1[ST];
+  __shared__ __attribute__((aligned(16))) uint8_t s2[ST];
+  __shared__ __attribute__((aligned(16))) uint8_t s3[ST];
+  __shared__ float xs_lds[PREC == 2 ? kQsMaxRows : 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t row0 = blockIdx.x * rows_per_block;
+  const uint32_t region = (blockIdx.y * gridDim.x + blockIdx.x) * kQsWaves + wave;
+  uint64_t* wemit = emit + size_t(region) * wave_cap;
+  if (row0 >= n) {  // the whole block: no rows
+    if (lane == 0) emit_count[region] = 0;
+    return;
+  }
+  const uint32_t nrf = (n + 15) >> 4;
+  const uint32_t rf0 = row0 >> 4, rf1 = min(nrf, rf0 + rows_per_block / 16);
+  const uint32_t nst = (rf1 - rf0 + SF - 1) / SF;
+
+  // this wave's query fragments: A operands, cuts, scales (fragments past the batch never emit)
+  const uint32_t nqf = (nq + 15) >> 4, q4 = 4 * (lane >> 4);
+  const FT* qsrc = reinterpret_cast<const FT*>(qfrag);
+  FT a[QW][KS];
+  float cut[QW][4], qsc[QW][4];
+#pragma unroll
+  for (int j = 0; j < QW; ++j) {
+    const uint32_t qf = (blockIdx.y * kQsWaves + wave) * QW + j;
+    const bool qok = qf < nqf;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a[j][s] = qok ? qsrc[(size_t(qf) * KS + s) * 64 + lane] : FT{};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cut[j][i] = qok ? qaux[qf * 32 + q4 + i] : __builtin_inff();
+      qsc[j][i] = (PREC == 2 && qok) ? qaux[qf * 32 + 16 + q4 + i] : 1.f;
+    }
+  }
+  if (PREC == 2)
+    for (uint32_t i = threadIdx.x; i < rows_per_block; i += 64 * kQsWaves)
+      xs_lds[i] = row0 + i < n ? xscale[row0 + i] : 1.f;
+  wait_vm<0>();
+  __syncthreads();
+
+  const uint8_t* xsrc = static_cast<const uint8_t*>(xfrag);
+  auto fetch = [&](uint8_t* dst, uint32_t st) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < LOADS; ++r) {
+      const uint32_t piece = uint32_t(wave + r * kQsWaves), fi = piece * 1024 / FB;
+      uint32_t rf = rf0 + st * SF + fi;
+      if (rf >= rf1) rf = rf0;  // past the block's rows: a fragment that is never read
+      load_lds16(xsrc + size_t(rf) * FB + (piece * 1024 - fi * FB) + lane * 16, dst + piece * 1024);
+    }
+  };
+  uint32_t n_emit = 0;  // wave-uniform
+  auto compute = [&](const uint8_t* stage, uint32_t st) __attribute__((always_inline)) {
+    const FT* bs = reinterpret_cast<const FT*>(stage);
+#pragma unroll
+    for (int fi = 0; fi < SF; ++fi) {
+      const uint32_t rf = rf0 + st * SF + fi;  // (past the block's rows: masked, not skipped)
+      f32x4m_t acc[QW];
+#pragma unroll
+      for (int j = 0; j < QW; ++j) acc[j] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const FT b = bs[(fi * KS + s) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < QW; ++j) acc[j] = mfma<PREC>(a[j][s], b, acc[j]);
+      }
+      // value i of acc[j] belongs to query (fragment of j) * 16 + q4 + i and row rf * 16 + lane % 16
+      const uint32_t row = rf * 16 + (lane & 15);
+      const bool ok = rf < rf1 && row < n;
+      const float xs = (PREC == 2 && ok) ? xs_lds[row - row0] : 1.f;
+      float sc[QW][4];
+      uint64_t any = 0;
+#pragma unroll
+      for (int j = 0; j < QW; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          sc[j][i] = PREC == 2 ? acc[j][i] * qsc[j][i] * xs : acc[j][i];
+          any |= ballot(ok && sc[j][i] >= cut[j][i]);
+        }
+      if (any) {
+#pragma unroll
+        for (int j = 0; j < QW; ++j)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const bool hit = ok && sc[j][i] >= cut[j][i];
+            const uint64_t m = ballot(hit);
+            if (m) {
+              const uint32_t pre =
+                  __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+              const uint32_t at = n_emit + pre;
+              const uint32_t q = ((blockIdx.y * kQsWaves + wave) * QW + j) * 16 + q4 + i;
+              if (hit && at < wave_cap) wemit[at] = make_key(1.0f - sc[j][i], (q << kQsShift) | (row - row0));
+              n_emit += uint32_t(__builtin_popcountll(m));
+            }
+          }
+      }
+    }
+  };
+  auto slot = [&](auto I) -> uint8_t* {
+    constexpr int v = decltype(I)::value % NBUF;
+    return v == 0 ? s0 : v == 1 ? s1 : v == 2 ? s2 : s3;
+  };
+  // consume stage st from slot I; refill slot I + NBUF - 1 (consumed at st - 1) with stage
+  // st + NBUF - 1 (past the last stage: the last stage again, into a slot nobody reads)
+  auto step = [&](auto I, uint32_t st) __attribute__((always_inline)) {
+    wait_vm<(NBUF - 2) * LOADS>();  // the later stages may stay in flight
+    ring_barrier();
+    fetch(slot(Slot<decltype(I)::value + NBUF - 1>()), min(st + NBUF - 1, nst - 1));
+    compute(slot(I), st);
+  };
+  fetch(s0, 0);
+  fetch(s1, min(1u, nst - 1));
+  fetch(s2, min(2u, nst - 1));
+  // The loop runs eight steps a trip: at the loop head the compiler waits for every load in
+  // flight (it does not carry its per-slot tracking of the LDS-DMA loads around the loop).
+  for (uint32_t st = 0;;) {  // leaves the loop right after the last stage
+    step(Slot<0>(), st);
+    if (++st == nst) break;
+    step(Slot<1>(), st);
+    if (++st == nst) break;
+    step(Slot<2>(), st);
+    if (++st == nst) break;
+    step(Slot<3>(), st);
+    if (++st == nst) break;
+    step(Slot<4>(), st);
+    if (++st == nst) break;
+    step(Slot<5>(), st);
+    if (++st == nst) break;
+    step(Slot<6>(), st);
+    if (++st == nst) break;
+    step(Slot<7>(), st);
+    if (++st == nst) break;
+  }
+  wait_vm<0>();  // the refetches land before the block's LDS is released
+  if (lane == 0) emit_count[region] = n_emit;
+}
+
+// Where the rows of an emission region start, and how an entry's low word splits into query
+// and row within the region: lo = (query << qshift) | row offset. Region r starts at row
+// ((r / waves) % blocks) * rows (rows-stationary pass 2: waves 1, one region per wave of
+// rows_frags * 16 rows, qshift 6; queries-stationary: 4 regions per block, one per wave, each
+// over the block's row range, qshift 20).
+struct RegionGeometry {
+  uint32_t waves, blocks, rows, qshift;
+  __device__ __forceinline__ uint32_t row(uint32_t r, uint32_t lo) const {
+    return ((r / waves) % blocks) * rows + (lo & ((1u << qshift) - 1u));
+  }
+};
+
 // Files the pass-2 entries of every wave region into per-query candidate lists (count[q] keys
 // (ord(dh) << 32) | row at cand[q * cap ..]) without global atomics on the per-query counters
-// (device-scope atomics on a few hundred hot addresses serialise); region r covers rows
-// r * rows_per_wave ... Three launches over S slices of `per` consecutive regions each:
+// (device-scope atomics on a few hundred hot addresses serialise). Three launches over S
+// slices of `per` consecutive regions each:
 //   count   per slice, entries per query in LDS -> slice_cnt[q * kScatterSlices + s]; a region that
 //           overflowed sets *overflow (the round is then answered by the exact scan);
 //   offsets per query (one wave), exclusive scan over the slices in place, total -> count[q];
@@ -444,8 +648,8 @@ constexpr uint32_t kScatterSlices = 1024;
 
 __global__ __launch_bounds__(256) void scatter_count_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
-    uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S, uint32_t* __restrict__ slice_cnt,
-    uint32_t* __restrict__ overflow) {
+    RegionGeometry geo, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
+    uint32_t* __restrict__ slice_cnt, uint32_t* __restrict__ overflow) {
   extern __shared__ uint32_t hist[];  // nq
   __shared__ uint32_t rc[64];
   const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -461,7 +665,7 @@ __global__ __launch_bounds__(256) void scatter_count_kernel(
         if (lane == 0) atomicAdd(overflow, 1u);
         continue;
       }
-      for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[uint32_t(emit[size_t(r) * wave_cap + i]) >> 6], 1u);
+      for (uint32_t i = lane; i < c; i += 64) atomicAdd(&hist[uint32_t(emit[size_t(r) * wave_cap + i]) >> geo.qshift], 1u);
     }
   }
   __syncthreads();
@@ -503,7 +707,7 @@ __global__ __launch_bounds__(256) void scatter_offsets_kernel(uint32_t* __restri
 
 __global__ __launch_bounds__(256) void scatter_place_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
-    uint32_t rows_per_wave, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
+    RegionGeometry geo, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
     const uint32_t* __restrict__ slice_off, uint64_t* __restrict__ cand, uint32_t cap) {
   extern __shared__ uint32_t base[];  // nq
   __shared__ uint32_t rc[64];
@@ -519,9 +723,9 @@ __global__ __launch_bounds__(256) void scatter_place_kernel(
       if (c > wave_cap) continue;
       for (uint32_t i = lane; i < c; i += 64) {
         const uint64_t e = emit[size_t(r) * wave_cap + i];
-        const uint32_t lo = uint32_t(e), q = lo >> 6;
+        const uint32_t lo = uint32_t(e), q = lo >> geo.qshift;
         const uint32_t slot = atomicAdd(&base[q], 1u);
-        if (slot < cap) cand[size_t(q) * cap + slot] = (e & 0xFFFFFFFF00000000ull) | (r * rows_per_wave + (lo & 63));
+        if (slot < cap) cand[size_t(q) * cap + slot] = (e & 0xFFFFFFFF00000000ull) | geo.row(r, lo);
       }
     }
   }
@@ -866,22 +1070,73 @@ int short_aux_pad() { return 64; }
 int short_waves() { return kWaves; }
 
 hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
-  const uint32_t n_regions = a.n_blocks * uint32_t(kWaves);
+  RegionGeometry geo;
+  uint32_t n_regions;
+  if (a.qs_blocks) {  // queries-stationary pass 2
+    n_regions = a.qs_blocks * a.qs_groups * uint32_t(kQsWaves);
+    geo = RegionGeometry{uint32_t(kQsWaves), a.qs_blocks, a.qs_rows, kQsShift};
+  } else {
+    n_regions = a.n_blocks * uint32_t(kWaves);
+    geo = RegionGeometry{1u, n_regions, uint32_t(rows_frags(prec, a.KS) * 16), 6u};
+  }
   if (n_regions == 0 || a.nq == 0) return hipErrorInvalidValue;  // count[] is written only below
   // about 16 regions a slice (4 a wave), at most kScatterSlices slices
   const uint32_t S = std::min(kScatterSlices, std::max(1u, (n_regions + 15) / 16));
   const uint32_t per = (n_regions + S - 1) / S;
   const size_t lds = size_t(a.nq) * 4;
-  hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap,
+  hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap, geo,
                      n_regions, per, a.nq, S, a.slice, a.overflow);
   hipLaunchKernelGGL(scatter_offsets_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, a.st, a.slice, S, a.nq,
                      a.count);
-  hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap,
-                     uint32_t(rows_frags(prec, a.KS) * 16), n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
+  hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap, geo,
+                     n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
   return hipGetLastError();
 }
 
 size_t short_scatter_slices() { return kScatterSlices; }
+
+namespace {
+template <int PREC, int KS>
+hipError_t launch_qs_t(const ShortScanArgs& a) {
+  hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS>), dim3(a.qs_blocks, a.qs_groups), dim3(64 * kQsWaves), 0, a.st,
+                     a.xfrag, a.xscale, a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.emit, a.wave_cap, a.emit_count);
+  return hipGetLastError();
+}
+template <int PREC>
+hipError_t launch_qs_p(const ShortScanArgs& a) {
+  switch (a.KS) {
+    case 2: return launch_qs_t<PREC, 2>(a);
+    case 4: return launch_qs_t<PREC, 4>(a);
+    case 8: return launch_qs_t<PREC, 8>(a);
+    case 12: return launch_qs_t<PREC, 12>(a);
+    case 24: return launch_qs_t<PREC, 24>(a);
+    default: return hipErrorInvalidValue;
+  }
+}
+}  // namespace
+
+bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint32_t* blocks, uint32_t* groups,
+                   uint32_t* rows, uint32_t* queries_per_region) {
+  if (ks != 2 && ks != 4 && ks != 8 && ks != 12 && ks != 24) return false;
+  const uint32_t qpg = uint32_t(kQsWaves * qs_qw(prec, ks) * 16);  // queries per query group
+  if (nq == 0 || nq > qpg || n == 0 || num_cus <= 0) return false;  // one query group
+  const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
+  const uint32_t nb = uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks));
+  uint32_t r = (n + nb - 1) / nb;
+  r = std::min(kQsMaxRows, (r + unit - 1) / unit * unit);
+  *rows = r;
+  *blocks = (n + r - 1) / r;
+  *groups = 1;
+  *queries_per_region = uint32_t(qs_qw(prec, ks)) * 16;
+  return true;
+}
+
+hipError_t launch_mfma_qs(int prec, const ShortScanArgs& a) {
+  if (a.nq == 0 || a.qs_blocks == 0 || a.qs_groups == 0 || a.qs_rows == 0 || a.qs_rows > kQsMaxRows ||
+      a.qs_rows % (16u * uint32_t(qs_sf(prec, a.KS))) != 0)
+    return hipErrorInvalidValue;
+  return prec == 1 ? launch_qs_p<1>(a) : launch_qs_p<2>(a);
+}
 
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
   if (!n) return hipSuccess;
