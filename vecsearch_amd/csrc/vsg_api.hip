@@ -184,7 +184,7 @@ struct vsg_index {
   float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
   float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
   int scan_waves_per_cu = 8;
-  int auto_short_batch = 32;
+  int auto_short_batch = 8;  // measured crossover at 1M x 384: the shortlist wins from 8 queries
   vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
   bool lists_autotune = true;
   std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class

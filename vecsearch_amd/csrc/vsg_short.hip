@@ -439,11 +439,12 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 // ahead). Every wave runs its fragments against every row
 // fragment, so HBM reads each row once per query group. The MFMA operands and their order are
 // those of mfma_scan_kernel, so the estimates are the same bits. Emission: wave w of block b,
-// query group y appends to its own region ((y * gridDim.x + b) * 4 + w) entries
+// query group y appends to its own region ((y * nbx + b) * 4 + w) entries
 // (ord(dh) << 32) | (query << kQsShift) | (row - first row of the block).
 constexpr int kQsWaves = 4;
 constexpr uint32_t kQsShift = 20;          // rows of a block < 2^20
 constexpr uint32_t kQsMaxRows = 4096;      // rows per block (FP8 scales of the block in LDS)
+constexpr uint32_t kQsMaxGroups = 4;       // query groups per launch (each reads the rows once)
 
 // row fragments per stage: the largest of 8, 4, 2, 1 whose stage fits 12 KB and splits into
 // whole 1 KB pieces, the same number for each wave; 1 if none fits (d = 768 in BF16: 24 KB).
@@ -480,7 +481,8 @@ template <int PREC, int KS>
 __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t rows_per_block,
-    uint64_t* __restrict__ emit, uint32_t wave_cap, uint32_t* __restrict__ emit_count) {
+    uint32_t nbx, uint32_t ngroups, uint64_t* __restrict__ emit, uint32_t wave_cap,
+    uint32_t* __restrict__ emit_count) {
   typedef typename Frag<PREC>::T FT;
   typedef QsShape<PREC, KS> SH;
   constexpr int SF = SH::SF, NBUF = SH::NBUF, QW = SH::QW, LOADS = SH::LOADS, ST = SH::STAGE;
@@ -493,8 +495,12 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
   __shared__ __attribute__((aligned(16))) uint8_t s3[ST];
   __shared__ float xs_lds[PREC == 2 ? kQsMaxRows : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint32_t row0 = blockIdx.x * rows_per_block;
-  const uint32_t region = (blockIdx.y * gridDim.x + blockIdx.x) * kQsWaves + wave;
+  // Block (bx, qg) of nbx x ngroups: the query groups of one row range are 8 workgroups apart,
+  // so they run on the same XCD (workgroups go round-robin over the 8 XCDs) at about the same
+  // time and the later ones read the rows from that XCD's L2. nbx is a multiple of 8.
+  const uint32_t bx = blockIdx.x / (8 * ngroups) * 8 + blockIdx.x % 8, qg = blockIdx.x / 8 % ngroups;
+  const uint32_t row0 = bx * rows_per_block;
+  const uint32_t region = (qg * nbx + bx) * kQsWaves + wave;
   uint64_t* wemit = emit + size_t(region) * wave_cap;
   if (row0 >= n) {  // the whole block: no rows
     if (lane == 0) emit_count[region] = 0;
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
   float cut[QW][4], qsc[QW][4];
 #pragma unroll
   for (int j = 0; j < QW; ++j) {
-    const uint32_t qf = (blockIdx.y * kQsWaves + wave) * QW + j;
+    const uint32_t qf = (qg * kQsWaves + wave) * QW + j;
     const bool qok = qf < nqf;
 #pragma unroll
     for (int s = 0; s < KS; ++s) a[j][s] = qok ? qsrc[(size_t(qf) * KS + s) * 64 + lane] : FT{};
@@ -576,7 +582,7 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
               const uint32_t pre =
                   __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
               const uint32_t at = n_emit + pre;
-              const uint32_t q = ((blockIdx.y * kQsWaves + wave) * QW + j) * 16 + q4 + i;
+              const uint32_t q = ((qg * kQsWaves + wave) * QW + j) * 16 + q4 + i;
               if (hit && at < wave_cap) wemit[at] = make_key(1.0f - sc[j][i], (q << kQsShift) | (row - row0));
               n_emit += uint32_t(__builtin_popcountll(m));
             }
@@ -1098,8 +1104,9 @@ size_t short_scatter_slices() { return kScatterSlices; }
 namespace {
 template <int PREC, int KS>
 hipError_t launch_qs_t(const ShortScanArgs& a) {
-  hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS>), dim3(a.qs_blocks, a.qs_groups), dim3(64 * kQsWaves), 0, a.st,
-                     a.xfrag, a.xscale, a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.emit, a.wave_cap, a.emit_count);
+  hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS>), dim3(a.qs_blocks * a.qs_groups), dim3(64 * kQsWaves), 0, a.st,
+                     a.xfrag, a.xscale, a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.qs_blocks, a.qs_groups, a.emit,
+                     a.wave_cap, a.emit_count);
   return hipGetLastError();
 }
 template <int PREC>
@@ -1119,14 +1126,17 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
                    uint32_t* rows, uint32_t* queries_per_region) {
   if (ks != 2 && ks != 4 && ks != 8 && ks != 12 && ks != 24) return false;
   const uint32_t qpg = uint32_t(kQsWaves * qs_qw(prec, ks) * 16);  // queries per query group
-  if (nq == 0 || nq > qpg || n == 0 || num_cus <= 0) return false;  // one query group
+  const uint32_t g = (nq + qpg - 1) / qpg;
+  if (nq == 0 || g > kQsMaxGroups || n == 0 || num_cus <= 0) return false;
+  // blocks for all groups: about qs_bpc per CU (one pass over the grid), rows a multiple of
+  // the stage's rows, row ranges a multiple of 8 (the XCD interleave of the groups)
   const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
-  const uint32_t nb = uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks));
+  const uint32_t nb = std::max(8u, uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks)) / g);
   uint32_t r = (n + nb - 1) / nb;
   r = std::min(kQsMaxRows, (r + unit - 1) / unit * unit);
   *rows = r;
-  *blocks = (n + r - 1) / r;
-  *groups = 1;
+  *blocks = ((n + r - 1) / r + 7) / 8 * 8;
+  *groups = g;
   *queries_per_region = uint32_t(qs_qw(prec, ks)) * 16;
   return true;
 }
