@@ -19,7 +19,9 @@ One JSON line on stdout carries the contract fields plus:
                 kernel against the dense MFMA peak (algorithmic flops 2*b*N*d per launch), the
                 id identity with the plain exact path, and the FP64 score error;
   sweep         per batch size: device-resident QPS of the plain exact path and of the two
-                shortlist forms, and the PCIe-inclusive QPS / p50 / p99 ms of method "auto".
+                shortlist forms (with the emission pass's time and its HBM fraction: the
+                low-precision base copy, N*d*2 or N*d bytes, over the pass), and the
+                PCIe-inclusive QPS / p50 / p99 ms of method "auto".
 """
 import argparse
 import json
@@ -478,7 +480,12 @@ def main():
         row["qps_plain"] = round(device_qps(b, "skinny", args.sweep_seconds)[0], 1)
         if b >= 8:
             for m in ("bf16", "fp8"):
-                row[f"qps_{m}"] = round(device_qps(b, m, args.sweep_seconds)[0], 1)
+                qps, kt_m = device_qps(b, m, args.sweep_seconds)
+                row[f"qps_{m}"] = round(qps, 1)
+                # the emission pass reads the low-precision base copy once: its HBM fraction
+                p2 = kt_m["mfma_ms"] / max(1, kt_m["mfma_launches"])
+                row[f"{m}_pass2_ms"] = round(p2, 4)
+                row[f"{m}_pass2_hbm_frac"] = round(n * d * (2 if m == "bf16" else 1) / (p2 / 1e3) / (HBM_PEAK_GBS * 1e9), 4)
         lat = []
         qh = queries
         while len(lat) < 3 or sum(lat) < args.sweep_seconds:
