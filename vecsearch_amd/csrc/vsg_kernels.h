@@ -43,10 +43,11 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
 }
 
 __device__ __forceinline__ uint64_t shfl_up1_64(uint64_t v, int lane) {
-  // value of lane-1 (lane 0 gets its own value; callers overwrite it)
-  const int src = (lane == 0 ? 0 : lane - 1) << 2;
-  const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, int(uint32_t(v)));
-  const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, int(uint32_t(v >> 32)));
+  // value of lane-1 (lane 0 keeps its own value; callers overwrite it): DPP wave_shr:1 (0x138),
+  // a VALU move, not an LDS round trip
+  (void)lane;
+  const uint32_t lo = uint32_t(__builtin_amdgcn_update_dpp(int(uint32_t(v)), int(uint32_t(v)), 0x138, 0xF, 0xF, false));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(int(uint32_t(v >> 32)), int(uint32_t(v >> 32)), 0x138, 0xF, 0xF, false));
   return (uint64_t(hi) << 32) | lo;
 }
 
