@@ -457,8 +457,14 @@ __host__ __device__ constexpr bool qs_sf_ok(int prec, int ks, int sf) {
 __host__ __device__ constexpr int qs_sf(int prec, int ks) {
   return qs_sf_ok(prec, ks, 8) ? 8 : qs_sf_ok(prec, ks, 4) ? 4 : qs_sf_ok(prec, ks, 2) ? 2 : 1;
 }
+// stages per row fragment: 2 where one fragment alone exceeds 12 KB (d = 768 in BF16: its k-steps
+// in two halves, the accumulators carried from the first to the second)
+__host__ __device__ constexpr int qs_hs(int prec, int ks) { return ks * (prec == 1 ? 1024 : 512) > (12 << 10) ? 2 : 1; }
+__host__ __device__ constexpr int qs_stage(int prec, int ks) {
+  return qs_sf(prec, ks) * ks * (prec == 1 ? 1024 : 512) / qs_hs(prec, ks);
+}
 __host__ __device__ constexpr int qs_lds(int prec, int ks) {  // the ring and, for FP8, the row scales
-  return 4 * qs_sf(prec, ks) * ks * (prec == 1 ? 1024 : 512) + (prec == 2 ? int(kQsMaxRows) * 4 : 0);
+  return 4 * qs_stage(prec, ks) + (prec == 2 ? int(kQsMaxRows) * 4 : 0);
 }
 __host__ __device__ constexpr int qs_bpc(int prec, int ks) {  // blocks per CU
   return (160 << 10) / qs_lds(prec, ks) > 3 ? 3 : (160 << 10) / qs_lds(prec, ks) < 1 ? 1 : (160 << 10) / qs_lds(prec, ks);
@@ -470,11 +476,14 @@ template <int PREC, int KS>
 struct QsShape {
   static constexpr int CB = Frag<PREC>::kChunkBytes;
   static constexpr int SF = qs_sf(PREC, KS);
-  static constexpr int STAGE = SF * KS * CB;
+  static constexpr int HS = qs_hs(PREC, KS);
+  static constexpr int STAGE = qs_stage(PREC, KS);
+  static constexpr int KH = KS / HS;                      // k-steps per stage
   static constexpr int NBUF = 4;
   static constexpr int LOADS = STAGE / 1024 / kQsWaves;  // DMA loads per wave per stage
   static constexpr int QW = qs_qw(PREC, KS);
   static_assert(STAGE % (1024 * kQsWaves) == 0 && (KS * CB) % 1024 == 0, "stage shape");
+  static_assert(HS == 1 || (SF == 1 && KS % HS == 0), "half stages hold one fragment");
 };
 
 template <int PREC, int KS>
@@ -485,7 +494,8 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
     uint32_t* __restrict__ emit_count) {
   typedef typename Frag<PREC>::T FT;
   typedef QsShape<PREC, KS> SH;
-  constexpr int SF = SH::SF, NBUF = SH::NBUF, QW = SH::QW, LOADS = SH::LOADS, ST = SH::STAGE;
+  constexpr int SF = SH::SF, HS = SH::HS, KH = SH::KH, NBUF = SH::NBUF, QW = SH::QW, LOADS = SH::LOADS,
+                ST = SH::STAGE;
   constexpr int FB = KS * SH::CB;  // bytes of one row fragment (all k-steps)
   // distinct LDS objects per slot: the compiler then waits only on the counted vmcnt
   static_assert(NBUF == 4, "four ring slots");
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
   }
   const uint32_t nrf = (n + 15) >> 4;
   const uint32_t rf0 = row0 >> 4, rf1 = min(nrf, rf0 + rows_per_block / 16);
-  const uint32_t nst = (rf1 - rf0 + SF - 1) / SF;
+  const uint32_t nst = (rf1 - rf0 + SF - 1) / SF * HS;  // stage st: fragments (st / HS) * SF .., part st % HS
 
   // this wave's query fragments: A operands, cuts, scales (fragments past the batch never emit)
   const uint32_t nqf = (nq + 15) >> 4, q4 = 4 * (lane >> 4);
@@ -535,29 +545,36 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
 
   const uint8_t* xsrc = static_cast<const uint8_t*>(xfrag);
   auto fetch = [&](uint8_t* dst, uint32_t st) __attribute__((always_inline)) {
+    const uint32_t fg = st / HS, part = st - fg * HS;
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
-      const uint32_t piece = uint32_t(wave + r * kQsWaves), fi = piece * 1024 / FB;
-      uint32_t rf = rf0 + st * SF + fi;
+      const uint32_t piece = uint32_t(wave + r * kQsWaves);
+      const uint32_t at = part * ST + piece * 1024, fi = at / FB;  // byte of the stage's fragments
+      uint32_t rf = rf0 + fg * SF + fi;
       if (rf >= rf1) rf = rf0;  // past the block's rows: a fragment that is never read
-      load_lds16(xsrc + size_t(rf) * FB + (piece * 1024 - fi * FB) + lane * 16, dst + piece * 1024);
+      load_lds16(xsrc + size_t(rf) * FB + (at - fi * FB) + lane * 16, dst + piece * 1024);
     }
   };
   uint32_t n_emit = 0;  // wave-uniform
-  auto compute = [&](const uint8_t* stage, uint32_t st) __attribute__((always_inline)) {
+  f32x4m_t acc[QW];     // carried from part 0 to part 1 of a fragment when HS = 2
+  // part: st % HS, known at compile time (the loop runs 8 steps a trip from a multiple of 8)
+  auto compute = [&](const uint8_t* stage, uint32_t st, auto part_c) __attribute__((always_inline)) {
+    constexpr int part = decltype(part_c)::value;
     const FT* bs = reinterpret_cast<const FT*>(stage);
 #pragma unroll
     for (int fi = 0; fi < SF; ++fi) {
-      const uint32_t rf = rf0 + st * SF + fi;  // (past the block's rows: masked, not skipped)
-      f32x4m_t acc[QW];
+      const uint32_t rf = rf0 + st / HS * SF + fi;  // (past the block's rows: masked, not skipped)
+      if (part == 0) {
 #pragma unroll
-      for (int j = 0; j < QW; ++j) acc[j] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const FT b = bs[(fi * KS + s) * 64 + lane];
-#pragma unroll
-        for (int j = 0; j < QW; ++j) acc[j] = mfma<PREC>(a[j][s], b, acc[j]);
+        for (int j = 0; j < QW; ++j) acc[j] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
       }
+#pragma unroll
+      for (int s = 0; s < KH; ++s) {
+        const FT b = bs[(fi * KH + s) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < QW; ++j) acc[j] = mfma<PREC>(a[j][part * KH + s], b, acc[j]);
+      }
+      if (part + 1 < HS) continue;  // the fragment's second half comes with the next stage
       // value i of acc[j] belongs to query (fragment of j) * 16 + q4 + i and row rf * 16 + lane % 16
       const uint32_t row = rf * 16 + (lane & 15);
       const bool ok = rf < rf1 && row < n;
@@ -600,7 +617,7 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
     wait_vm<(NBUF - 2) * LOADS>();  // the later stages may stay in flight
     ring_barrier();
     fetch(slot(Slot<decltype(I)::value + NBUF - 1>()), min(st + NBUF - 1, nst - 1));
-    compute(slot(I), st);
+    compute(slot(I), st, Slot<decltype(I)::value % HS>());
   };
   fetch(s0, 0);
   fetch(s1, min(1u, nst - 1));
