@@ -307,6 +307,23 @@ def test_shortlist_small_batch_shapes(n, d, nq):
         assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
 
 
+@pytest.mark.parametrize("n", [5, 65_537, 65_551])
+@pytest.mark.parametrize("nq", [3, 129, 300])
+def test_shortlist_ragged_and_tiny_bases(n, nq):
+    # the shortlist serves bases of 65,536 rows and up: one row past a whole fragment, 15 past
+    # (a partial last fragment), with one and several query groups of the queries-stationary
+    # pass 2 and the rows-stationary one (300 queries); a 5-row base takes the exact path, and
+    # k > n pads with -1 / inf
+    d = 128
+    x = dg.mixture(n, d, seed=92)
+    q = dg.mixture(nq, d, seed=93)
+    gpu = vs.GpuBruteForceIndex(d, "cosine")
+    gpu.add(x)
+    for method in ("bf16", "fp8"):
+        for k in (1, 10):
+            assert_same(gpu.search(q, k, method=method), oracle.search(x, q, k, "ip", threads=8))
+
+
 @pytest.mark.parametrize("d", [100, 128, 256, 768])
 def test_shortlist_other_dims(d):
     x = dg.mixture(66_000, d, seed=42)
