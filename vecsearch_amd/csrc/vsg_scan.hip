@@ -210,13 +210,11 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
       }
       __syncthreads();
       T = red[0];
-    } else {  // bitonic sort: log2(n2)^2 / 2 barrier-separated stages
-      uint32_t n2 = 256;
-      while (n2 < ns) n2 <<= 1;
-      for (uint32_t i = ns + tid; i < n2; i += 256) buf[i] = kPadKey;
-      __syncthreads();
-      bitonic_sort(buf, n2, tid);
-      T = buf[kq - 1];
+    } else {  // radix select of the kq-th smallest high word (the distance bits): every key with
+              // that high word or a smaller one is kept, at least kq of them
+      __shared__ uint32_t hist[256], sel[2];
+      const uint32_t hi = block_select_hi32(buf, ns, kq, hist, sel, tid);
+      T = (uint64_t(hi) << 32) | 0xFFFFFFFFull;
     }
     __syncthreads();
   }

@@ -913,51 +913,6 @@ __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
     }
 }
 
-// The kk-th smallest (1-based, 1 <= kk <= c) of the high words of buf[0, c) (the order-preserving
-// bits of the estimates), by a 256-thread block: radix select, 8 bits a pass; wave 0 finds
-// each pass's bin with a prefix sum over the 256 bins (4 a lane). Uses hist (256) and sel (2).
-__device__ uint32_t block_select_hi32(const uint64_t* buf, uint32_t c, uint32_t kk, uint32_t* hist, uint32_t* sel,
-                                      uint32_t tid) {
-  uint32_t prefix = 0, pmask = 0, rank = kk;
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[tid] = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < c; i += 256) {
-      const uint32_t x = uint32_t(buf[i] >> 32);
-      if ((x & pmask) == prefix) atomicAdd(&hist[(x >> shift) & 255], 1u);
-    }
-    __syncthreads();
-    if (tid < 64) {
-      uint32_t h[4], sum = 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sum += (h[j] = hist[4 * tid + j]);
-      uint32_t incl = sum;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off, 64);
-        if (tid >= uint32_t(off)) incl += y;
-      }
-      uint32_t acc = incl - sum;
-      if (acc < rank && rank <= incl) {  // exactly one lane
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (acc + h[j] >= rank) {
-            sel[0] = 4 * tid + j;
-            sel[1] = rank - acc;
-            break;
-          }
-          acc += h[j];
-        }
-      }
-    }
-    __syncthreads();
-    prefix |= sel[0] << shift;
-    pmask |= 255u << shift;
-    rank = sel[1];
-  }
-  return prefix;
-}
-
 // Handles the queries with cmin <= count <= CAP; with final_pass set, flags those above CAP.
 template <int D16, bool L2, int CAP>
 __global__ __launch_bounds__(256) void rescore_kernel(
