@@ -20,6 +20,10 @@
 // Error bound E (inner product; all norms computed in FP64 when the copies are built):
 //   |sum q x - sum q^ x^| <= |q| |x - x^| + |q - q^| |x^|           (exact identity + C-S)
 //   MFMA accumulation (products of bf16 / fp8 are exact in fp32)   <= 2 gamma_d |q^| |x^|
+//   FP8 only: the 16x16x32 FP8 instruction's own sum of its 32      <= 2^-10 |q^| |x^|
+//     products is measured at up to 2^-12.4 of their absolute sum (not the fp32 accuracy the
+//     term above assumes); 2^-10 of sum |q^_i x^_i| <= 2^-10 |q^| |x^| covers it 5x
+//     (tests/gpu_probes/mfma_accuracy.hip, run by the GPU tests)
 //   FP8 per-vector scales: two extra fp32 roundings                 <= 4 u |q^| |x^|
 //   reference CPU order vs the exact sum                            <= gamma_m |q| |x|
 //   the final 1 - s roundings on both sides                         <= 4 u (1 + |q| |x|)
@@ -150,7 +154,7 @@ __device__ __forceinline__ float query_error_bound(double qx2, double qh2, doubl
   const double gd = dd * u / (1 - dd * u), gm = m * u / (1 - m * u);
   double e = nq_ * dX + dq * Xh;
   e += 2.0 * gd * nqh * Xh;
-  if (prec == 2) e += 4.0 * u * nqh * Xh;
+  if (prec == 2) e += (4.0 * u + 0x1p-10) * nqh * Xh;  // scales; the FP8 instruction's own sum
   e += gm * nq_ * X;
   e += 4.0 * u * (1.0 + nq_ * X);
   return float(e * (1.0 + 1e-6) + 1e-12);
