@@ -111,7 +111,7 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
 
 /* Launch-shape parameters. Known names:
  *   "scan_waves_per_cu"    waves per CU of the streaming scan (1..32, default 8);
- *   "auto_short_batch"     smallest batch that AUTO sends to the shortlist (default 8);
+ *   "auto_short_batch"     smallest batch that AUTO sends to the shortlist (default 2);
  *   "lists_qb", "lists_r", "lists_chunk", "lists_blocks_per_cu", "lists_lanes_per_row"
  *                          a fixed tile of the filtered scan (queries per work item 1/2/4/8/16,
  *                          row groups per lane step 2/4, allowed rows per item, blocks per CU,
@@ -173,7 +173,8 @@ int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t
 
 /* ---- captured searches (no counterpart in the reference) ----
  * vsg_graph_create records one exact search of nq device-resident queries (the streaming scan
- * and its merge, method NAIVE / SKINNY / AUTO below the shortlist batch) into a HIP graph with
+ * and its merge, method NAIVE / SKINNY / AUTO, which captures the streaming scan at any batch
+ * size; the shortlist methods are refused) into a HIP graph with
  * its own workspace; vsg_graph_launch replays it on `stream`, reading whatever d_queries holds
  * at that point and writing d_ids / d_dists. The buffers and the index must outlive the graph.
  * The graph is bound to the base's size and address at capture: after vsg_add, a launch

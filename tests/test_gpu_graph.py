@@ -1,5 +1,6 @@
 """Captured exact searches (vsg_graph_*): every replay equals the eager search of the queries the
-buffer holds at that moment, with eager work of the index between the replays."""
+buffer holds at that moment, with eager work of the index between the replays. (AUTO captures
+the streaming scan; eager AUTO of 13 queries takes the shortlist: the same bits.)"""
 import numpy as np
 import pytest
 
@@ -14,7 +15,7 @@ def bits(a):
     return np.asarray(a, np.float32).view(np.uint32)
 
 
-@pytest.mark.parametrize("nq,k,method", [(1, 10, "naive"), (8, 10, "skinny"), (13, 100, "skinny")])
+@pytest.mark.parametrize("nq,k,method", [(1, 10, "naive"), (8, 10, "skinny"), (13, 100, "skinny"), (13, 10, "auto")])
 def test_graph_replays_match_eager(nq, k, method):
     import torch
 

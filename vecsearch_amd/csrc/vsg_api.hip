@@ -184,7 +184,7 @@ struct vsg_index {
   float* base = nullptr;   // cap x dp, row-major, zero padded (gathers: filters, rescoring)
   float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
   int scan_waves_per_cu = 8;
-  int auto_short_batch = 8;  // measured crossover at 1M x 384: the shortlist wins from 8 queries
+  int auto_short_batch = 2;  // measured at 1M x 384: the shortlist is faster from 2 queries (and at 1)
   vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
   bool lists_autotune = true;
   std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class
@@ -1456,8 +1456,8 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
     return fail(VSG_ERR_INVALID_ARGUMENT, "a captured search needs nq > 0, k > 0 and device buffers");
   std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
-  if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16 || method == VSG_METHOD_FP8 ||
-      (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch) && short_supported(h, k)))
+  // AUTO captures the exact streaming scan at any batch size
+  if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16 || method == VSG_METHOD_FP8)
     return fail(VSG_ERR_INVALID_ARGUMENT,
                 "only the exact streaming scan can be captured (the shortlist reads its overflow count on the host)");
   if (h->n == 0) return fail(VSG_ERR_INVALID_ARGUMENT, "cannot capture a search of an empty index");
