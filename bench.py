@@ -447,6 +447,26 @@ def main():
         alg_bytes = int(np.sum(sizes[used])) * d * 4 + args.filtered * d * 4 + int(np.sum(fks)) * 8
         flops = 2.0 * d * pairs
         lat = np.array(lat)
+        # the same batch captured once (tuned tile, filters fixed) and replayed on device queries
+        fdq = torch.from_numpy(fq).to(dev)
+        fdi = torch.empty((args.filtered, 100), dtype=torch.int32, device=dev)
+        fdd = torch.empty((args.filtered, 100), dtype=torch.float32, device=dev)
+        fg_graph = vs.SearchGraph.lists(idx, fdq.data_ptr(), fks, franges, gids, fdi.data_ptr(),
+                                        fdd.data_ptr(), kmax=100)
+        fg_graph.launch(sptr)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        greps = 10
+        for _ in range(greps):
+            fg_graph.launch(sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        fgraph_ms = e0.elapsed_time(e1) / greps
+        fg_same = bool(np.array_equal(np.where(fdi.cpu().numpy().view(np.uint32) == 0xFFFFFFFF, -1,
+                                               fdi.cpu().numpy().view(np.uint32).astype(np.int64)), fres[0])
+                       and np.array_equal(fdd.cpu().numpy().view(np.uint32), fres[1].view(np.uint32)))
+        fg_graph.close()
+        del fdq, fdi, fdd
         filtered = {
             "workload": f"{nf:,} x {d} rows (the first rows of the base), 100 Zipf(s=1.1) groups; "
                         f"{args.filtered} queries, each one uniform group, k uniform in {{10, 100}}",
@@ -456,6 +476,9 @@ def main():
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "first_call_s": round(tune_s, 3),
+            "graph_ms_per_step": round(fgraph_ms, 4),
+            "graph_qps": round(args.filtered / (fgraph_ms / 1e3), 1),
+            "graph_bit_identical_to_eager": fg_same,
             "tile": {"qb": idx.get_param("lists_last_qb"), "r": idx.get_param("lists_last_r"),
                      "chunk": idx.get_param("lists_last_chunk"),
                      "lanes_per_row": idx.get_param("lists_last_lanes_per_row")},

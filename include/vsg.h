@@ -182,6 +182,16 @@ int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t
 typedef struct vsg_graph vsg_graph;
 int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
                      uint32_t* d_ids, float* d_dists, vsg_graph** out);
+/* vsg_graph_create_lists records one persistent launch of the batched filtered scan
+ * (vsg_search_lists' arguments: per-query ks, ranges into allow_ids, VSG_ALL_ROWS for an
+ * unfiltered query) over nq device-resident queries, writing nq x kmax results to d_ids /
+ * d_dists. ks, ranges and allowed ids are host buffers, planned and copied into the graph's
+ * workspace at capture (a replay reads new queries, not new filters). The tile is the one an
+ * eager call would use: the fixed tile, else the tuned tile of the batch's shape class, else
+ * the planner's. Replays are kernels only. */
+int vsg_graph_create_lists(vsg_index* h, const float* d_queries, size_t nq, const uint32_t* ks, size_t kmax,
+                           const uint64_t* range, const uint32_t* allow_ids, size_t n_ids, uint32_t* d_ids,
+                           float* d_dists, vsg_graph** out);
 int vsg_graph_launch(vsg_graph* g, void* stream);
 void vsg_graph_destroy(vsg_graph* g);
 

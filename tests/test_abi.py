@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(vs.LIBRARY_PATH)
     names = declared_symbols()
     assert {"vsg_create", "vsg_add", "vsg_search", "vsg_search_lists", "vsg_last_error", "vsg_get_param",
-            "vsg_graph_create", "vsg_graph_launch", "vsg_server_create", "vsg_server_submit",
+            "vsg_graph_create", "vsg_graph_create_lists", "vsg_graph_launch", "vsg_server_create", "vsg_server_submit",
             "vsg_server_wait"} <= names
     for name in sorted(names):
         assert hasattr(lib, name), name
@@ -51,6 +51,7 @@ def test_device_count_and_errors_without_compute():
     assert lib.vsg_server_wait(None, ctypes.c_uint64(1), None) == 1
     g = ctypes.c_void_p()
     assert lib.vsg_graph_create(None, None, 1, 1, 0, None, None, ctypes.byref(g)) == 1
+    assert lib.vsg_graph_create_lists(None, None, 1, None, 1, None, None, 0, None, None, ctypes.byref(g)) == 1
     assert lib.vsg_graph_launch(None, None) == 1
     v = ctypes.c_int64()
     assert lib.vsg_get_param(None, b"lists_qb", ctypes.byref(v)) == 1

@@ -1,5 +1,6 @@
-"""Captured exact searches (vsg_graph_*): every replay equals the eager search of the queries the
-buffer holds at that moment, with eager work of the index between the replays. (AUTO captures
+"""Captured exact searches (vsg_graph_*, vsg_graph_create_lists): every replay equals the eager
+search of the queries the buffer holds at that moment, with eager work of the index between the
+replays. (AUTO captures
 the streaming scan; eager AUTO of 13 queries takes the shortlist: the same bits.)"""
 import numpy as np
 import pytest
@@ -64,6 +65,80 @@ def test_graph_limits():
     g.launch(0)
     gpu.synchronize()
     gpu.add(x[:16])  # the index grew: the captured plan no longer covers it
+    with pytest.raises(RuntimeError):
+        g.launch(0)
+    g.close()
+
+
+@pytest.mark.parametrize("metric,tile", [("ip", None), ("l2", None), ("ip", (32, 8, 1024, 8)), ("ip", (16, 2, 2048, 8))])
+def test_lists_graph_replays_match_eager(metric, tile):
+    # one persistent launch of the filtered scan (filtered and unfiltered queries, k in {1, 10, 100})
+    # captured once; each replay reads new queries under the filters fixed at capture
+    import torch
+
+    n, d, nq = 40_000, 384, 120
+    x = dg.mixture(n, d, seed=73)
+    groups = dg.zipf_groups(n, 30, 1.1, seed=74)
+    off, ids_g = dg.group_csr(groups, 30)
+    ks, qg = dg.query_filters(nq, 30, seed=75, ks=(1, 10, 100))
+    ranges = dg.query_ranges(qg, off)
+    ranges[::4] = [vs.ALL_ROWS, 0]
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    if tile:
+        for name, v in zip(("lists_qb", "lists_r", "lists_chunk", "lists_lanes_per_row"), tile):
+            gpu.set_param(name, v)
+    dev = torch.device("cuda:0")
+    d_q = torch.empty((nq, d), dtype=torch.float32, device=dev)
+    d_ids = torch.empty((nq, 100), dtype=torch.int32, device=dev)
+    d_dists = torch.empty((nq, 100), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    graph = vs.SearchGraph.lists(gpu, d_q.data_ptr(), ks, ranges, ids_g, d_ids.data_ptr(), d_dists.data_ptr(),
+                                 kmax=100)
+    for rep in range(3):
+        q = dg.mixture(nq, d, seed=90 + rep)
+        d_q.copy_(torch.from_numpy(q))
+        torch.cuda.synchronize()
+        graph.launch(stream.cuda_stream)
+        stream.synchronize()
+        got_ids = d_ids.cpu().numpy().view(np.uint32)
+        got_d = d_dists.cpu().numpy()
+        ei, ed = gpu.search_batch(q, ks, ranges, ids_g, kmax=100)  # eager work between replays
+        np.testing.assert_array_equal(np.where(got_ids == oracle.NONE, -1, got_ids.astype(np.int64)), ei)
+        np.testing.assert_array_equal(bits(got_d), bits(ed))
+        if rep == 0:
+            oi, od = oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=100)
+            np.testing.assert_array_equal(got_ids, oi)
+            np.testing.assert_array_equal(bits(got_d), bits(od))
+    graph.close()
+
+
+def test_lists_graph_limits():
+    import torch
+
+    d = 64
+    x = dg.mixture(5_000, d, seed=76)
+    gpu = vs.GpuBruteForceIndex(d, "cosine")
+    gpu.add(x)
+    dev = torch.device("cuda:0")
+    d_q = torch.zeros((4, d), dtype=torch.float32, device=dev)
+    d_ids = torch.empty((4, 10), dtype=torch.int32, device=dev)
+    d_dists = torch.empty((4, 10), dtype=torch.float32, device=dev)
+    ks = np.full(4, 10, np.uint32)
+    ranges = np.array([[0, 2]] * 4, np.uint64)
+    with pytest.raises(ValueError):  # allowed ids must be increasing and < size()
+        vs.SearchGraph.lists(gpu, d_q.data_ptr(), ks, ranges, np.array([7, 3], np.uint32), d_ids.data_ptr(),
+                             d_dists.data_ptr())
+    with pytest.raises(ValueError):  # k > 128
+        vs.SearchGraph.lists(gpu, d_q.data_ptr(), np.full(4, 200, np.uint32), ranges,
+                             np.array([3, 7], np.uint32), d_ids.data_ptr(), d_dists.data_ptr())
+    g = vs.SearchGraph.lists(gpu, d_q.data_ptr(), ks, ranges, np.array([3, 7], np.uint32), d_ids.data_ptr(),
+                             d_dists.data_ptr())
+    g.launch(0)
+    gpu.synchronize()
+    ids = d_ids.cpu().numpy().view(np.uint32)
+    assert set(ids[:, :2].ravel()) == {3, 7} and (ids[:, 2:] == oracle.NONE).all()
+    gpu.add(x[:16])  # the index grew
     with pytest.raises(RuntimeError):
         g.launch(0)
     g.close()

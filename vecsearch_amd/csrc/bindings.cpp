@@ -85,6 +85,24 @@ class PyGraph {
                          reinterpret_cast<float*>(d_dists), &g_) != VSG_OK)
       throw std::invalid_argument(vsg_last_error());
   }
+  // A captured filtered batch (vsg_graph_create_lists); ks / ranges / allow_ids as search_batch.
+  PyGraph(py::object index, uintptr_t d_q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,
+          py::array_t<uint64_t, py::array::c_style | py::array::forcecast> ranges,
+          py::array_t<uint32_t, py::array::c_style | py::array::forcecast> allow_ids, size_t kmax, uintptr_t d_ids,
+          uintptr_t d_dists)
+      : index_(std::move(index)) {
+    auto& idx = index_.cast<GpuBruteForceIndex&>();
+    if (ks.ndim() != 1) throw std::invalid_argument("ks must be a 1-D array with one k per query");
+    const size_t nq = size_t(ks.shape(0));
+    if (ranges.ndim() != 2 || size_t(ranges.shape(0)) != nq || ranges.shape(1) != 2)
+      throw std::invalid_argument("ranges must have shape (nq, 2): [begin, end) into allow_ids");
+    if (allow_ids.ndim() != 1) throw std::invalid_argument("allow_ids must be a 1-D array");
+    for (size_t i = 0; i < nq; ++i) kmax = std::max<size_t>(kmax, ks.data()[i]);
+    if (vsg_graph_create_lists(idx.handle(), reinterpret_cast<const float*>(d_q), nq, ks.data(), kmax,
+                               ranges.data(), allow_ids.data(), size_t(allow_ids.shape(0)),
+                               reinterpret_cast<uint32_t*>(d_ids), reinterpret_cast<float*>(d_dists), &g_) != VSG_OK)
+      throw std::invalid_argument(vsg_last_error());
+  }
   ~PyGraph() { close(); }
   void launch(uintptr_t stream) {
     if (!g_) throw std::runtime_error("graph is closed");
@@ -419,6 +437,19 @@ PYBIND11_MODULE(_vecsearch, m) {
            py::arg("index"), py::arg("d_queries"), py::arg("nq"), py::arg("k"), py::arg("d_ids"),
            py::arg("d_dists"), py::arg("method") = "skinny",
            "Capture one exact search of nq device-resident queries into a HIP graph (vsg_graph_create).")
+      .def_static(
+          "lists",
+          [](py::object index, uintptr_t d_q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,
+             py::array_t<uint64_t, py::array::c_style | py::array::forcecast> ranges,
+             py::array_t<uint32_t, py::array::c_style | py::array::forcecast> allow_ids, uintptr_t d_ids,
+             uintptr_t d_dists, size_t kmax) {
+            return new PyGraph(std::move(index), d_q, ks, ranges, allow_ids, kmax, d_ids, d_dists);
+          },
+          py::arg("index"), py::arg("d_queries"), py::arg("ks"), py::arg("ranges"), py::arg("allow_ids"),
+          py::arg("d_ids"), py::arg("d_dists"), py::arg("kmax") = 0,
+          "Capture one persistent launch of the batched filtered scan (search_batch's ks, ranges and\n"
+          "allow_ids, fixed at capture) over len(ks) device-resident queries; results are\n"
+          "len(ks) x kmax (kmax >= max(ks)) in d_ids (uint32) / d_dists (vsg_graph_create_lists).")
       .def("launch", &PyGraph::launch, py::arg("stream") = 0, "Replay the captured search on a HIP stream.")
       .def("close", &PyGraph::close);
 

@@ -679,6 +679,49 @@ int ilog2_floor(uint64_t v) {
   return r;
 }
 
+// Shape class of a filtered batch in the tuned-tile table (dim, k > 64, queries per distinct
+// allowed set, rows per distinct allowed set, both log2), and its allowed (query, row) pairs.
+struct ListsClass {
+  std::tuple<size_t, int, int, int> key;
+  uint64_t pairs = 0;
+};
+ListsClass lists_class(const vsg_index* h, size_t nq, size_t kmax, const uint64_t* range) {
+  std::set<std::pair<uint64_t, uint64_t>> distinct;
+  uint64_t pairs = 0, rows = 0;
+  for (size_t i = 0; i < nq; ++i) {
+    const bool all = range[2 * i] == VSG_ALL_ROWS;
+    const uint64_t len = all ? h->n : (range[2 * i + 1] >= range[2 * i] ? range[2 * i + 1] - range[2 * i] : 0);
+    pairs += len;
+    const std::pair<uint64_t, uint64_t> rk(all ? uint64_t(VSG_ALL_ROWS) : range[2 * i],
+                                           all ? uint64_t(0) : range[2 * i + 1]);
+    if (distinct.insert(rk).second) rows += len;
+  }
+  const size_t nd = std::max<size_t>(1, distinct.size());
+  ListsClass c;
+  c.key = std::make_tuple(h->dim, kmax > 64 ? 1 : 0, ilog2_floor(nq / nd + 1), ilog2_floor(rows / nd + 1));
+  c.pairs = pairs;
+  return c;
+}
+
+bool lookup_tuned(vsg_index* h, const std::tuple<size_t, int, int, int>& key, vsg::ListsTiles* out) {
+  std::lock_guard<std::mutex> g(h->tune_mu);
+  auto it = h->lists_tuned.find(key);
+  if (it == h->lists_tuned.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+// The tile an eager call of this batch runs with when it makes no candidate runs: the fixed
+// tile, else the tuned tile of its shape class, else the planner's.
+vsg::ListsTiles lists_tiles_for(vsg_index* h, size_t nq, size_t kmax, const uint64_t* range) {
+  const vsg::ListsTiles& fixed = h->lists_tiles;
+  const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0 &&
+                       fixed.lanes_per_row == 0;
+  vsg::ListsTiles t = fixed;
+  if (planner && h->lists_autotune) (void)lookup_tuned(h, lists_class(h, nq, kmax, range).key, &t);
+  return t;
+}
+
 // Filtered / mixed batch through the persistent lists scan, with the tile autotuner; host
 // buffers in and out, on stream st with workspace w. The caller holds h->rw shared.
 // allow_tune = false (serving): an untuned class runs the planner's tile instead of pausing the
@@ -691,29 +734,11 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
   const bool planner = fixed.qb == 0 && fixed.r == 0 && fixed.chunk == 0 && fixed.blocks_per_cu == 0 &&
                        fixed.lanes_per_row == 0;
   if (planner && h->lists_autotune) {
-    std::set<std::pair<uint64_t, uint64_t>> distinct;
-    uint64_t pairs = 0, rows = 0;
-    for (size_t i = 0; i < nq; ++i) {
-      const bool all = range[2 * i] == VSG_ALL_ROWS;
-      const uint64_t len = all ? h->n : (range[2 * i + 1] >= range[2 * i] ? range[2 * i + 1] - range[2 * i] : 0);
-      pairs += len;
-      const std::pair<uint64_t, uint64_t> rk(all ? uint64_t(VSG_ALL_ROWS) : range[2 * i],
-                                             all ? uint64_t(0) : range[2 * i + 1]);
-      if (distinct.insert(rk).second) rows += len;
-    }
-    const size_t nd = std::max<size_t>(1, distinct.size());
-    const auto key = std::make_tuple(h->dim, kmax > 64 ? 1 : 0, ilog2_floor(nq / nd + 1),
-                                     ilog2_floor(rows / nd + 1));
-    bool have = false;
+    const ListsClass cls = lists_class(h, nq, kmax, range);
+    const auto& key = cls.key;
+    const uint64_t pairs = cls.pairs;
     vsg::ListsTiles tuned;
-    {
-      std::lock_guard<std::mutex> g(h->tune_mu);
-      auto it = h->lists_tuned.find(key);
-      if (it != h->lists_tuned.end()) {
-        have = true;
-        tuned = it->second;
-      }
-    }
+    const bool have = lookup_tuned(h, key, &tuned);
     if (have)
       return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus, st,
                                     queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists,
@@ -1489,6 +1514,50 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
   e = hipStreamEndCapture(gr->st, &graph);
   gr->graph = graph;
   if (rc) return cleanup(rc);
+  if (e != hipSuccess) return cleanup(fail_hip("hipStreamEndCapture", e));
+  e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return cleanup(fail_hip("hipGraphInstantiate", e));
+  *out = gr;
+  return VSG_OK;
+}
+
+int vsg_graph_create_lists(vsg_index* h, const float* d_queries, size_t nq, const uint32_t* ks, size_t kmax,
+                           const uint64_t* range, const uint32_t* allow_ids, size_t n_ids, uint32_t* d_ids,
+                           float* d_dists, vsg_graph** out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
+  if (kmax > VSG_MAX_K) return fail(VSG_ERR_INVALID_ARGUMENT, "k must be <= " + std::to_string(VSG_MAX_K));
+  if (nq == 0 || kmax == 0 || !d_queries || !ks || !range || !d_ids || !d_dists)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "a captured search needs nq > 0, kmax > 0 and device buffers");
+  std::lock_guard<std::mutex> g(h->mu);
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  if (h->n == 0) return fail(VSG_ERR_INVALID_ARGUMENT, "cannot capture a search of an empty index");
+  VSG_HIP(hipSetDevice(h->device));
+  vsg_graph* gr = new (std::nothrow) vsg_graph;
+  if (!gr) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
+  gr->h = h;
+  gr->n = h->n;
+  gr->base = h->base;
+  auto cleanup = [&](int code) {
+    vsg_graph_destroy(gr);
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&gr->st, hipStreamNonBlocking);
+  if (e != hipSuccess) return cleanup(fail_hip("vsg_graph_create_lists", e));
+  // plan and upload before capture: the graph holds the kernels only, reading the plan, ks and
+  // allowed ids from its own workspace
+  vsg::ListsLaunch L;
+  int rc = vsg::prepare_lists_device(gr->w.lists, h->n, h->dim, h->dp, h->metric, h->num_cus, gr->st, nq, ks, kmax,
+                                     range, allow_ids, n_ids, g_err, lists_tiles_for(h, nq, kmax, range), L);
+  if (rc) return cleanup(rc);
+  e = hipStreamBeginCapture(gr->st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return cleanup(fail_hip("hipStreamBeginCapture", e));
+  const hipError_t le = vsg::lists_enqueue(gr->w.lists, L, h->base, d_queries, d_ids, d_dists, gr->st, nullptr);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(gr->st, &graph);
+  gr->graph = graph;
+  if (le != hipSuccess) return cleanup(fail_hip("vsg_graph_create_lists", le));
   if (e != hipSuccess) return cleanup(fail_hip("hipStreamEndCapture", e));
   e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
   if (e != hipSuccess) return cleanup(fail_hip("hipGraphInstantiate", e));

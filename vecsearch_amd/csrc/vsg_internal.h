@@ -125,4 +125,22 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
                       float* out_dists, std::string& err, TimingSink* timing,
                       const ListsTiles& tiles, ListsTiles* used);
 
+// Launch shape of a planned filtered batch (its plan, ks and allowed ids already in the
+// workspace).
+struct ListsLaunch {
+  uint32_t nq = 0, kmax = 0, n_items = 0, grid = 0, dp = 0, d = 0;
+  size_t lds = 0;
+  int qb = 0, r = 0, lpr = 0, kp = 0;
+  bool l2 = false;
+};
+// Plans a filtered batch of nq device-resident queries and uploads its plan into ws; returns
+// once the uploads are complete. The batch's kernels are then recorded by lists_enqueue.
+int prepare_lists_device(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,
+                         hipStream_t st, size_t nq, const uint32_t* ks, size_t kmax, const uint64_t* range,
+                         const uint32_t* allow_ids, size_t n_ids, std::string& err, const ListsTiles& tiles,
+                         ListsLaunch& out);
+// Kernels only (no memset, copy or allocation): queries d_q (nq x d), results nq x kmax.
+hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
+                         uint32_t* d_ids, float* d_dists, hipStream_t st, TimingSink* timing);
+
 }  // namespace vsg

@@ -483,12 +483,21 @@ void ListsWorkspace::release() {
     }                                          \
   } while (0)
 
-int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t dim, size_t dp,
-                      int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
-                      const uint32_t* ks, size_t kmax, const uint64_t* range,
-                      const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
-                      float* out_dists, std::string& err, TimingSink* timing,
-                      const ListsTiles& tiles, ListsTiles* used) {
+namespace {
+
+// Host-side plan of a filtered batch: kept alive until the uploads that read it are complete.
+struct ListsPlanHost {
+  std::vector<ListItem> items;
+  std::vector<uint32_t> qperm;
+  std::vector<uint64_t> slot_off;
+};
+
+// Validates and plans a filtered batch, reserves its workspace and uploads the plan, ks and
+// the allowed ids on st (and the host queries, when given). The uploads read `ph`.
+int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,
+                  hipStream_t st, const float* queries, size_t nq, const uint32_t* ks, size_t kmax,
+                  const uint64_t* range, const uint32_t* allow_ids, size_t n_ids, std::string& err,
+                  const ListsTiles& tiles, ListsTiles* used, ListsPlanHost& ph, ListsLaunch& out) {
   // ---- validate (hnsw.cpp:326: an empty allowed set returns nothing) ----
   const uint64_t total = n_ids;
   for (size_t i = 0; i < nq; ++i) {
@@ -584,10 +593,13 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
     used->chunk = C;
     used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
   }
-  std::vector<ListItem> items;
-  std::vector<uint32_t> qperm;
+  std::vector<ListItem>& items = ph.items;
+  std::vector<uint32_t>& qperm = ph.qperm;
+  std::vector<uint64_t>& slot_off = ph.slot_off;
+  items.clear();
+  qperm.clear();
   qperm.reserve(nq);
-  std::vector<uint64_t> slot_off(nq + 1, 0);
+  slot_off.assign(nq + 1, 0);
   std::vector<uint32_t> nchunks_of(nq, 0);
   // Items run chunk-major within a group: the query blocks of one chunk are pulled one after
   // another, so the chunk's rows are re-read from cache rather than from HBM.
@@ -618,22 +630,26 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
 
   // ---- upload ----
   const size_t qbytes = nq * dim * sizeof(float);
-  LISTS_HIP(dev_reserve(ws, kDQ, qbytes));
+  if (queries) {
+    LISTS_HIP(dev_reserve(ws, kDQ, qbytes));
+    LISTS_HIP(dev_reserve(ws, kDOut, nq * kmax * (sizeof(uint32_t) + sizeof(float))));
+    LISTS_HIP(host_reserve(ws, 0, qbytes));
+    LISTS_HIP(host_reserve(ws, 1, nq * kmax * sizeof(uint32_t)));
+    LISTS_HIP(host_reserve(ws, 2, nq * kmax * sizeof(float)));
+  }
   LISTS_HIP(dev_reserve(ws, kDKs, nq * sizeof(uint32_t)));
   LISTS_HIP(dev_reserve(ws, kDItems, std::max<size_t>(1, items.size()) * sizeof(ListItem)));
   LISTS_HIP(dev_reserve(ws, kDQperm, nq * sizeof(uint32_t)));
   LISTS_HIP(dev_reserve(ws, kDSlot, (nq + 1) * sizeof(uint64_t)));
   LISTS_HIP(dev_reserve(ws, kDIds, std::max<uint64_t>(1, total) * sizeof(uint32_t)));
   LISTS_HIP(dev_reserve(ws, kDPart, std::max<uint64_t>(1, n_slots) * L * sizeof(uint64_t)));
-  LISTS_HIP(dev_reserve(ws, kDOut, nq * kmax * (sizeof(uint32_t) + sizeof(float))));
   LISTS_HIP(dev_reserve(ws, kDThr, nq * sizeof(uint64_t)));
-  LISTS_HIP(host_reserve(ws, 0, qbytes));
-  LISTS_HIP(host_reserve(ws, 1, nq * kmax * sizeof(uint32_t)));
-  LISTS_HIP(host_reserve(ws, 2, nq * kmax * sizeof(float)));
   if (!ws.counter) LISTS_HIP(hipMalloc(&ws.counter, sizeof(uint32_t)));
 
-  std::memcpy(ws.host[0], queries, qbytes);
-  LISTS_HIP(hipMemcpyAsync(ws.dev[kDQ], ws.host[0], qbytes, hipMemcpyHostToDevice, st));
+  if (queries) {
+    std::memcpy(ws.host[0], queries, qbytes);
+    LISTS_HIP(hipMemcpyAsync(ws.dev[kDQ], ws.host[0], qbytes, hipMemcpyHostToDevice, st));
+  }
   LISTS_HIP(hipMemcpyAsync(ws.dev[kDKs], ks, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   if (!items.empty())
     LISTS_HIP(hipMemcpyAsync(ws.dev[kDItems], items.data(), items.size() * sizeof(ListItem),
@@ -643,49 +659,102 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
                            hipMemcpyHostToDevice, st));
   if (total)
     LISTS_HIP(hipMemcpyAsync(ws.dev[kDIds], allow_ids, total * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  LISTS_HIP(hipMemsetAsync(ws.counter, 0, sizeof(uint32_t), st));
-  LISTS_HIP(hipMemsetAsync(ws.dev[kDThr], 0xFF, nq * sizeof(uint64_t), st));  // kPadKey
 
-  uint32_t* d_ids = static_cast<uint32_t*>(ws.dev[kDOut]);
-  float* d_dists = reinterpret_cast<float*>(d_ids + nq * kmax);
-  if (!items.empty()) {
+  const uint64_t want_blocks = block ? items.size() : (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int bpc = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
+  out.nq = uint32_t(nq);
+  out.kmax = uint32_t(kmax);
+  out.n_items = uint32_t(items.size());
+  out.grid = uint32_t(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * bpc));
+  out.lds = block ? 0 : size_t(kWavesPerBlock) * qb * dp * sizeof(float);
+  out.dp = uint32_t(dp);
+  out.d = uint32_t(dim);
+  out.qb = qb;
+  out.r = r;
+  out.lpr = lpr;
+  out.kp = kmax <= 64 ? 64 : 128;
+  out.l2 = metric == VSG_METRIC_L2;
+  return VSG_OK;
+}
+
+// counter = 0 and every per-query threshold = the pad key (a kernel, so that a captured step
+// holds no memset node)
+__global__ void lists_reset_kernel(uint32_t* counter, uint64_t* qthr, uint32_t nq) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *counter = 0u;
+  if (i < nq) qthr[i] = ~uint64_t(0);
+}
+
+}  // namespace
+
+hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
+                         uint32_t* d_ids, float* d_dists, hipStream_t st, TimingSink* timing) {
+  hipLaunchKernelGGL(lists_reset_kernel, dim3((L.nq + 255) / 256 + (L.nq == 0)), dim3(256), 0, st, ws.counter,
+                     static_cast<uint64_t*>(ws.dev[kDThr]), L.nq);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (L.n_items) {
     ListsArgs a;
-    const uint64_t want_blocks = block ? items.size() : (items.size() + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int bpc = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
-    a.grid = dim3(unsigned(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * bpc)));
-    a.lds = block ? 0 : size_t(kWavesPerBlock) * qb * dp * sizeof(float);
+    a.grid = dim3(L.grid);
+    a.lds = L.lds;
     a.st = st;
     a.base = base;
-    a.dp = uint32_t(dp);
-    a.d = uint32_t(dim);
-    a.q = static_cast<const float*>(ws.dev[kDQ]);
+    a.dp = L.dp;
+    a.d = L.d;
+    a.q = d_q;
     a.ks = static_cast<const uint32_t*>(ws.dev[kDKs]);
     a.items = static_cast<const ListItem*>(ws.dev[kDItems]);
-    a.n_items = uint32_t(items.size());
+    a.n_items = L.n_items;
     a.qperm = static_cast<const uint32_t*>(ws.dev[kDQperm]);
     a.ids = static_cast<const uint32_t*>(ws.dev[kDIds]);
     a.slot_off = static_cast<const uint64_t*>(ws.dev[kDSlot]);
-    a.L = L;
+    a.L = L.kmax;
     a.part = static_cast<uint64_t*>(ws.dev[kDPart]);
     a.counter = ws.counter;
     a.qthr = static_cast<uint64_t*>(ws.dev[kDThr]);
-    const bool l2 = metric == VSG_METRIC_L2;
-    const int kp = kmax <= 64 ? 64 : 128;
     if (timing) timing->begin(st);
-    LISTS_HIP(launch_lists_tile(qb, r, lpr, l2, kp, a));
+    if ((e = launch_lists_tile(L.qb, L.r, L.lpr, L.l2, L.kp, a)) != hipSuccess) return e;
     if (timing) timing->end(st, VSG_KERNEL_LISTS, 1);
   }
   if (timing) timing->begin(st);
-  LISTS_HIP(launch_merge(uint32_t(nq), static_cast<const uint64_t*>(ws.dev[kDPart]),
-                         static_cast<const uint64_t*>(ws.dev[kDSlot]), 0, L,
-                         static_cast<const uint32_t*>(ws.dev[kDKs]), 0, nullptr, d_ids, d_dists,
-                         uint32_t(kmax), st));
+  e = launch_merge(L.nq, static_cast<const uint64_t*>(ws.dev[kDPart]), static_cast<const uint64_t*>(ws.dev[kDSlot]),
+                   0, L.kmax, static_cast<const uint32_t*>(ws.dev[kDKs]), 0, nullptr, d_ids, d_dists, L.kmax, st);
+  if (e != hipSuccess) return e;
   if (timing) timing->end(st, VSG_KERNEL_MERGE, 1);
+  return hipSuccess;
+}
+
+int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t dim, size_t dp,
+                      int metric, int num_cus, hipStream_t st, const float* queries, size_t nq,
+                      const uint32_t* ks, size_t kmax, const uint64_t* range,
+                      const uint32_t* allow_ids, size_t n_ids, uint32_t* out_ids,
+                      float* out_dists, std::string& err, TimingSink* timing,
+                      const ListsTiles& tiles, ListsTiles* used) {
+  ListsPlanHost ph;
+  ListsLaunch L;
+  const int rc = lists_prepare(ws, n, dim, dp, metric, num_cus, st, queries, nq, ks, kmax, range, allow_ids,
+                               n_ids, err, tiles, used, ph, L);
+  if (rc) return rc;
+  uint32_t* d_ids = static_cast<uint32_t*>(ws.dev[kDOut]);
+  float* d_dists = reinterpret_cast<float*>(d_ids + nq * kmax);
+  LISTS_HIP(lists_enqueue(ws, L, base, static_cast<const float*>(ws.dev[kDQ]), d_ids, d_dists, st, timing));
   LISTS_HIP(hipMemcpyAsync(ws.host[1], d_ids, nq * kmax * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   LISTS_HIP(hipMemcpyAsync(ws.host[2], d_dists, nq * kmax * sizeof(float), hipMemcpyDeviceToHost, st));
   LISTS_HIP(hipStreamSynchronize(st));
   std::memcpy(out_ids, ws.host[1], nq * kmax * sizeof(uint32_t));
   std::memcpy(out_dists, ws.host[2], nq * kmax * sizeof(float));
+  return VSG_OK;
+}
+
+int prepare_lists_device(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,
+                         hipStream_t st, size_t nq, const uint32_t* ks, size_t kmax, const uint64_t* range,
+                         const uint32_t* allow_ids, size_t n_ids, std::string& err, const ListsTiles& tiles,
+                         ListsLaunch& out) {
+  ListsPlanHost ph;
+  const int rc = lists_prepare(ws, n, dim, dp, metric, num_cus, st, nullptr, nq, ks, kmax, range, allow_ids,
+                               n_ids, err, tiles, nullptr, ph, out);
+  if (rc) return rc;
+  LISTS_HIP(hipStreamSynchronize(st));  // the plan's host vectors are read by the uploads
   return VSG_OK;
 }
 
