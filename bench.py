@@ -17,11 +17,12 @@ One JSON line on stdout carries the contract fields plus:
                 oracle timed on this host's cores over a bounded sample of the same workload;
   shortlist     the BF16 and FP8 certified-shortlist forms at a large batch: QPS, the MFMA scan
                 kernel against the dense MFMA peak (algorithmic flops 2*b*N*d per launch), the
-                id identity with the plain exact path, and the FP64 score error;
+                id identity with the plain exact path, the FP64 score error, and the QPS of the
+                same search captured into a HIP graph and replayed back to back;
   sweep         per batch size: device-resident QPS of the plain exact path and of the two
                 shortlist forms (with the emission pass's time and its HBM fraction: the
-                low-precision base copy, N*d*2 or N*d bytes, over the pass), and the
-                PCIe-inclusive QPS / p50 / p99 ms of method "auto".
+                low-precision base copy, N*d*2 or N*d bytes, over the pass; and replayed from
+                a captured graph), and the PCIe-inclusive QPS / p50 / p99 ms of method "auto".
 """
 import argparse
 import json
@@ -342,7 +343,8 @@ def main():
     roof["merge_ms"] = round(kt["merge_ms"] / max(1, kt["merge_launches"]), 4)
 
     def device_qps(b, method, seconds):
-        """QPS of method at batch b, queries and results in HBM; kernel times of the same calls."""
+        """QPS of method at batch b, queries and results in HBM (no event timing inside the
+        timed calls); then per-kernel times from a second, instrumented run of the same calls."""
         step_m(0, b, method)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -350,16 +352,39 @@ def main():
         torch.cuda.synchronize()
         one = time.perf_counter() - t0
         reps = int(min(200, max(2, seconds / max(one, 1e-6))))
-        idx.kernel_timing(True)
-        idx.kernel_times()
         e0.record(stream)
         for r in range(reps):
             step_m(r, b, method)
         e1.record(stream)
         torch.cuda.synchronize()
+        qps = b * reps / (e0.elapsed_time(e1) / 1e3)
+        idx.kernel_timing(True)
+        idx.kernel_times()
+        for r in range(min(reps, 20)):
+            step_m(r, b, method)
+        torch.cuda.synchronize()
         kt = idx.kernel_times()
         idx.kernel_timing(False)
-        return b * reps / (e0.elapsed_time(e1) / 1e3), kt
+        return qps, kt
+
+    def graph_qps(b, method, seconds):
+        """QPS of the same search captured once into a HIP graph and replayed back to back
+        (queries and results in HBM; no host work between the replays)."""
+        g = vs.SearchGraph(idx, dq.data_ptr(), b, k, ids.data_ptr(), dists.data_ptr(), method=method)
+        g.launch(sptr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.launch(sptr)
+        torch.cuda.synchronize()
+        one = time.perf_counter() - t0
+        reps = int(min(500, max(3, seconds / max(one, 1e-6))))
+        e0.record(stream)
+        for _ in range(reps):
+            g.launch(sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        g.close()
+        return b * reps / (e0.elapsed_time(e1) / 1e3)
 
     def step_m(i, b, method):
         off = (i * b) % (nqt - b + 1)
@@ -412,6 +437,8 @@ def main():
                 "dists_bit_identical_to_plain": bool(np.array_equal(got_d.view(np.uint32), plain_d.view(np.uint32))),
                 "checked_queries": nchk,
             }
+            shortlist[m]["graph_qps"] = round(graph_qps(sb, m, args.short_seconds), 1)
+            shortlist[m]["graph_ids_identical_to_plain"] = bool(np.array_equal(ids[:nchk].cpu().numpy(), plain_ids))
             log(f"shortlist {m}: {shortlist[m]}")
         shortlist["fp64_max_abs_err"] = fp64_error(queries[:nchk], plain_ids, plain_d)
 
@@ -505,6 +532,7 @@ def main():
             for m in ("bf16", "fp8"):
                 qps, kt_m = device_qps(b, m, args.sweep_seconds)
                 row[f"qps_{m}"] = round(qps, 1)
+                row[f"qps_{m}_graph"] = round(graph_qps(b, m, args.sweep_seconds), 1)
                 # the emission pass reads the low-precision base copy once: its HBM fraction
                 p2 = kt_m["mfma_ms"] / max(1, kt_m["mfma_launches"])
                 row[f"{m}_pass2_ms"] = round(p2, 4)

@@ -59,8 +59,8 @@ def test_graph_limits():
     d_q = torch.zeros((64, d), dtype=torch.float32, device=dev)
     d_ids = torch.empty((64, 10), dtype=torch.int32, device=dev)
     d_dists = torch.empty((64, 10), dtype=torch.float32, device=dev)
-    with pytest.raises(ValueError):  # the shortlist is not capturable
-        vs.SearchGraph(gpu, d_q.data_ptr(), 64, 10, d_ids.data_ptr(), d_dists.data_ptr(), method="bf16")
+    with pytest.raises(ValueError):  # the legacy tiled method is not capturable
+        vs.SearchGraph(gpu, d_q.data_ptr(), 64, 10, d_ids.data_ptr(), d_dists.data_ptr(), method="tiled")
     g = vs.SearchGraph(gpu, d_q.data_ptr(), 64, 10, d_ids.data_ptr(), d_dists.data_ptr(), method="skinny")
     g.launch(0)
     gpu.synchronize()
@@ -142,3 +142,81 @@ def test_lists_graph_limits():
     with pytest.raises(RuntimeError):
         g.launch(0)
     g.close()
+
+
+def _replay_and_check(gpu, x, make_q, nq, k, method, reps=3, metric="ip"):
+    import torch
+
+    d = x.shape[1]
+    dev = torch.device("cuda:0")
+    d_q = torch.empty((nq, d), dtype=torch.float32, device=dev)
+    d_ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d_dists = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    graph = vs.SearchGraph(gpu, d_q.data_ptr(), nq, k, d_ids.data_ptr(), d_dists.data_ptr(), method=method)
+    for rep in range(reps):
+        q = make_q(rep)
+        d_q.copy_(torch.from_numpy(q))
+        torch.cuda.synchronize()
+        graph.launch(stream.cuda_stream)
+        stream.synchronize()
+        got_ids = d_ids.cpu().numpy().view(np.uint32)
+        got_d = d_dists.cpu().numpy()
+        ei, ed = gpu.search(q, k, method=method)  # eager work between replays
+        np.testing.assert_array_equal(np.where(got_ids == oracle.NONE, -1, got_ids.astype(np.int64)), ei)
+        np.testing.assert_array_equal(bits(got_d), bits(ed))
+        if rep == 0:
+            oi, od = oracle.search(x, q, k, metric, threads=8)
+            np.testing.assert_array_equal(got_ids, oi)
+            np.testing.assert_array_equal(bits(got_d), bits(od))
+    graph.close()
+
+
+@pytest.mark.parametrize("nq,k,method", [(13, 10, "bf16"), (128, 10, "bf16"), (300, 100, "bf16"),
+                                         (128, 10, "fp8"), (64, 10, "auto")])
+def test_shortlist_graph_replays_match_eager(nq, k, method):
+    x = dg.mixture(80_000, 384, seed=77)
+    gpu = vs.GpuBruteForceIndex(384, "cosine")
+    gpu.add(x)
+    _replay_and_check(gpu, x, lambda rep: dg.mixture(nq, 384, seed=100 + rep), nq, k, method)
+
+
+def test_shortlist_graph_candidate_overflow():
+    # 9,000 tied rows overflow the candidate buffer of the queries near them: inside the graph
+    # those queries are answered by the device-planned exact fallback, the others by the shortlist
+    x = dg.mixture(70_000, 128, seed=44)
+    x[10_000:19_000] = x[5]
+    gpu = vs.GpuBruteForceIndex(128, "cosine")
+    gpu.add(x)
+
+    def make_q(rep):
+        near = x[5:6] * 0.999 + 0.001 * dg.mixture(4, 128, seed=46 + rep)
+        q = np.concatenate([near[:1], dg.mixture(40, 128, seed=45 + rep), near[1:], dg.mixture(20, 128, seed=47 + rep)])
+        return np.ascontiguousarray(vs.normalize(q.astype(np.float32)), dtype=np.float32)
+
+    _replay_and_check(gpu, x, make_q, 64, 50, "bf16")
+    gpu.search(make_q(0), 50, method="bf16")
+    assert gpu.short_stats()["fallbacks"] >= 4
+
+
+def test_shortlist_region_overflow_eager_and_graph():
+    # 600 queries (the rows-stationary pass 2) all near 3,000 tied rows: the tiles holding those
+    # rows emit more candidates than their regions hold, and the whole batch is answered by the
+    # exact scan (eager: on the host's decision; captured: by the device-planned fallback of
+    # every query)
+    x = dg.mixture(70_000, 128, seed=50)
+    x[20_000:23_000] = x[7]
+    gpu = vs.GpuBruteForceIndex(128, "cosine")
+    gpu.add(x)
+
+    def make_q(rep):
+        q = x[7:8] * 0.999 + 0.001 * dg.mixture(600, 128, seed=51 + rep)
+        return np.ascontiguousarray(vs.normalize(q.astype(np.float32)), dtype=np.float32)
+
+    q = make_q(0)
+    got = gpu.search(q, 10, method="bf16")
+    assert gpu.short_stats()["fallbacks"] == 600
+    oi, od = oracle.search(x, q, 10, "ip", threads=8)
+    np.testing.assert_array_equal(got[0], oi.astype(np.int64))
+    np.testing.assert_array_equal(bits(got[1]), bits(od))
+    _replay_and_check(gpu, x, make_q, 600, 10, "bf16", reps=2)

@@ -73,7 +73,7 @@ py::array_t<int64_t> ids_to_int64(const py::array_t<uint32_t>& ids) {
   return out;
 }
 
-// A captured exact search (vsg_graph_*): raw device pointers, replayed by launch().
+// A captured search (vsg_graph_*): raw device pointers, replayed by launch().
 class PyGraph {
  public:
   PyGraph(py::object index, uintptr_t d_q, size_t nq, size_t k, uintptr_t d_ids, uintptr_t d_dists,
@@ -436,7 +436,9 @@ PYBIND11_MODULE(_vecsearch, m) {
       .def(py::init<py::object, uintptr_t, size_t, size_t, uintptr_t, uintptr_t, const std::string&>(),
            py::arg("index"), py::arg("d_queries"), py::arg("nq"), py::arg("k"), py::arg("d_ids"),
            py::arg("d_dists"), py::arg("method") = "skinny",
-           "Capture one exact search of nq device-resident queries into a HIP graph (vsg_graph_create).")
+           "Capture one search of nq device-resident queries into a HIP graph (vsg_graph_create): the\n"
+           "streaming scan, or the certified shortlist (bf16 / fp8 / auto from auto_short_batch queries)\n"
+           "with its exact fallback planned on the device; method 'tiled' is refused.")
       .def_static(
           "lists",
           [](py::object index, uintptr_t d_q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,

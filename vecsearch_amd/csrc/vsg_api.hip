@@ -445,14 +445,17 @@ int ensure_prec(vsg_index* h, int p) {
 int enqueue_search(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
                    const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st);
 
-// Shortlist search of nq device-resident queries (prec 1 bf16, 2 fp8). Synchronises the stream
-// once at the end to learn whether any query overflowed its candidate buffer; those queries are
-// answered again by the exact scan.
-int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int p, uint32_t* d_ids,
-                  float* d_dists, hipStream_t st) {
-  int rc = ensure_prec(h, p);
-  if (rc) return rc;
-  const auto& c = h->prec[p];
+// Launch shape of a shortlist search and its workspace (short_reserve).
+struct ShortPlan {
+  uint32_t cap = 0, tile_rows = 0, n_tiles = 0, wave_cap = 0;
+  size_t round = 0, qpad = 0;
+  bool qs = false;
+  uint32_t qs_blocks = 0, qs_groups = 0, qs_rows = 0, qs_qpr = 0;
+};
+
+// Plans a shortlist search of nq queries and sizes the workspace for it (no allocation when it
+// already fits, as inside a captured step).
+int short_reserve(vsg_index* h, Workspace& w, size_t nq, size_t k, int p, ShortPlan& sp) {
   const int ks = h->ks;
   const uint32_t cap = uint32_t(vsg::short_cand_cap());
   const uint32_t tile_rows = uint32_t(vsg::short_tile_rows_for(p, ks));
@@ -493,6 +496,38 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
   if ((e = w.s_emit_count.reserve(n_regions)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_slice.reserve(round * vsg::short_scatter_slices())) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_hover.reserve(2)) != hipSuccess) return fail_hip("pinned staging", e);
+  sp.cap = cap;
+  sp.tile_rows = tile_rows;
+  sp.n_tiles = n_tiles;
+  sp.wave_cap = wave_cap;
+  sp.round = round;
+  sp.qpad = qpad;
+  sp.qs = qs;
+  sp.qs_blocks = qs_blocks;
+  sp.qs_groups = qs_groups;
+  sp.qs_rows = qs_rows;
+  sp.qs_qpr = qs_qpr;
+  return VSG_OK;
+}
+
+// Shortlist search of nq device-resident queries (prec 1 bf16, 2 fp8). Without `dev_fallback`
+// it synchronises the stream once per round to learn whether any query overflowed its
+// candidate buffer (or an emission region overflowed); those queries are answered again by the
+// exact scan. With it (a captured step), every round ends with the device-planned exact
+// fallback of vsg::lists_fallback_enqueue instead, and nothing waits on the host.
+int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int p, uint32_t* d_ids,
+                  float* d_dists, hipStream_t st, const vsg::ListsLaunch* dev_fallback = nullptr) {
+  int rc = ensure_prec(h, p);
+  if (rc) return rc;
+  const auto& c = h->prec[p];
+  const int ks = h->ks;
+  ShortPlan sp;
+  if ((rc = short_reserve(h, w, nq, k, p, sp))) return rc;
+  const uint32_t cap = sp.cap, n_tiles = sp.n_tiles, wave_cap = sp.wave_cap;
+  const size_t round = sp.round, qpad = sp.qpad;
+  const bool qs = sp.qs;
+  const uint32_t qs_blocks = sp.qs_blocks, qs_groups = sp.qs_groups, qs_rows = sp.qs_rows;
+  hipError_t e;
   w.st_emitted = w.st_rescored = w.st_fallback = 0;
   w.st_nq = 0;
 
@@ -563,6 +598,12 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
                                   qr, rq, nullptr, uint32_t(k), w.s_E.p, w.s_cand.p, w.s_count.p, ids,
                                   dists, uint32_t(k), w.s_flags.p, w.s_nover.p, w.s_rescored.p));
       ts.launches++;
+    }
+    if (dev_fallback) {
+      VSG_HIP(vsg::lists_fallback_enqueue(w.lists, *dev_fallback, h->base, qr, rq, w.s_flags.p, w.s_nover.p + 1,
+                                          ids, dists, uint32_t(k), st));
+      w.st_nq += rq;
+      continue;
     }
     VSG_HIP(hipMemcpyAsync(w.s_hover.p, w.s_nover.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VSG_HIP(hipStreamSynchronize(st));
@@ -1481,11 +1522,16 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
     return fail(VSG_ERR_INVALID_ARGUMENT, "a captured search needs nq > 0, k > 0 and device buffers");
   std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
-  // AUTO captures the exact streaming scan at any batch size
-  if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16 || method == VSG_METHOD_FP8)
-    return fail(VSG_ERR_INVALID_ARGUMENT,
-                "only the exact streaming scan can be captured (the shortlist reads its overflow count on the host)");
+  if (method == VSG_METHOD_TILED)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "the tiled method cannot be captured (use AUTO, NAIVE, SKINNY, BF16 or FP8)");
   if (h->n == 0) return fail(VSG_ERR_INVALID_ARGUMENT, "cannot capture a search of an empty index");
+  // the path an eager call takes (enqueue_search): the shortlist (p > 0) or the streaming scan
+  int p = 0;
+  if (short_supported(h, k)) {
+    if (method == VSG_METHOD_BF16) p = 1;
+    else if (method == VSG_METHOD_FP8) p = 2;
+    else if (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch)) p = 1;
+  }
   VSG_HIP(hipSetDevice(h->device));
   vsg_graph* gr = new (std::nothrow) vsg_graph;
   if (!gr) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -1499,16 +1545,31 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
   hipError_t e = hipStreamCreateWithFlags(&gr->st, hipStreamNonBlocking);
   if (e != hipSuccess) return cleanup(fail_hip("vsg_graph_create", e));
   // size the workspace before capture (no allocation may happen inside it)
-  const ScanPlan p = plan_scan(h);
-  if ((e = gr->w.part.reserve(std::min<size_t>(kPartBudgetBytes / sizeof(uint64_t),
-                                               ((nq + 7) / 8 * 8) * size_t(p.W) * k))) != hipSuccess)
-    return cleanup(fail_hip("workspace", e));
+  vsg::ListsLaunch fallback;
+  if (p) {
+    // the low-precision copy, the shortlist's buffers, and the device-planned exact fallback
+    // (a filtered-scan launch sized for a whole round of queries)
+    if ((rc = ensure_prec(h, p))) return cleanup(rc);
+    ShortPlan sp;
+    if ((rc = short_reserve(h, gr->w, nq, k, p, sp))) return cleanup(rc);
+    if ((rc = vsg::prepare_lists_fallback(gr->w.lists, h->n, h->dim, h->dp, h->metric, h->num_cus, gr->st,
+                                          sp.round, k, g_err, fallback)))
+      return cleanup(rc);
+  } else {
+    const ScanPlan sc = plan_scan(h);
+    if ((e = gr->w.part.reserve(std::min<size_t>(kPartBudgetBytes / sizeof(uint64_t),
+                                                 ((nq + 7) / 8 * 8) * size_t(sc.W) * k))) != hipSuccess)
+      return cleanup(fail_hip("workspace", e));
+  }
   e = hipStreamBeginCapture(gr->st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return cleanup(fail_hip("hipStreamBeginCapture", e));
   const bool timing = h->timing;
   h->timing = false;  // no event records inside the graph
-  rc = enqueue_exact(h, gr->w, d_queries, nq, k, method == VSG_METHOD_AUTO ? VSG_METHOD_SKINNY : method,
-                     nullptr, d_ids, d_dists, gr->st);
+  if (p)
+    rc = enqueue_short(h, gr->w, d_queries, nq, k, p, d_ids, d_dists, gr->st, &fallback);
+  else
+    rc = enqueue_exact(h, gr->w, d_queries, nq, k, method == VSG_METHOD_AUTO ? VSG_METHOD_SKINNY : method,
+                       nullptr, d_ids, d_dists, gr->st);
   h->timing = timing;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(gr->st, &graph);

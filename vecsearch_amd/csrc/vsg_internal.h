@@ -91,9 +91,9 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {
-  void* dev[9] = {};      // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out,
-                          // per-query thresholds)
-  size_t dev_cap[9] = {};
+  void* dev[10] = {};     // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out,
+                          // per-query thresholds, merge query map)
+  size_t dev_cap[10] = {};
   void* host[3] = {};     // pinned staging (queries, ids out, dists out)
   size_t host_cap[3] = {};
   uint32_t* counter = nullptr;
@@ -132,6 +132,8 @@ struct ListsLaunch {
   size_t lds = 0;
   int qb = 0, r = 0, lpr = 0, kp = 0;
   bool l2 = false;
+  uint64_t chunk = 0;          // rows per work item
+  uint32_t rows = 0, nch = 0;  // an unfiltered batch: base rows, chunks per query
 };
 // Plans a filtered batch of nq device-resident queries and uploads its plan into ws; returns
 // once the uploads are complete. The batch's kernels are then recorded by lists_enqueue.
@@ -142,5 +144,15 @@ int prepare_lists_device(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, in
 // Kernels only (no memset, copy or allocation): queries d_q (nq x d), results nq x kmax.
 hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
                          uint32_t* d_ids, float* d_dists, hipStream_t st, TimingSink* timing);
+// The exact answer of a device-chosen subset of up to nq unfiltered queries, k results each:
+// the filtered scan planned at capture for all nq queries (prepare_lists_fallback); at run time
+// a one-block kernel picks the queries i < rq with flags[i] != 0 (every one when *all != 0) and
+// writes their work items, then the scan and the merge write those queries' rows of
+// d_ids / d_dists (row stride ldo). Kernels only.
+int prepare_lists_fallback(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,
+                           hipStream_t st, size_t nq, size_t k, std::string& err, ListsLaunch& out);
+hipError_t lists_fallback_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
+                                  uint32_t rq, const uint32_t* flags, const uint32_t* all, uint32_t* d_ids,
+                                  float* d_dists, uint32_t ldo, hipStream_t st);
 
 }  // namespace vsg

@@ -322,7 +322,7 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
 
 namespace {
 
-enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut, kDThr };
+enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut, kDThr, kDQmap };
 
 hipError_t dev_reserve(ListsWorkspace& ws, int i, size_t bytes) {
   if (bytes <= ws.dev_cap[i]) return hipSuccess;
@@ -460,7 +460,7 @@ bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
 }
 
 void ListsWorkspace::release() {
-  for (int i = 0; i < 9; ++i) {
+  for (int i = 0; i < 10; ++i) {
     if (dev[i]) (void)hipFree(dev[i]);
     dev[i] = nullptr;
     dev_cap[i] = 0;
@@ -674,6 +674,7 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   out.lpr = lpr;
   out.kp = kmax <= 64 ? 64 : 128;
   out.l2 = metric == VSG_METRIC_L2;
+  out.chunk = C;
   return VSG_OK;
 }
 
@@ -685,6 +686,86 @@ __global__ void lists_reset_kernel(uint32_t* counter, uint64_t* qthr, uint32_t n
   if (i < nq) qthr[i] = ~uint64_t(0);
 }
 
+ListsArgs lists_args(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q, hipStream_t st) {
+  ListsArgs a;
+  a.grid = dim3(L.grid);
+  a.lds = L.lds;
+  a.st = st;
+  a.base = base;
+  a.dp = L.dp;
+  a.d = L.d;
+  a.q = d_q;
+  a.ks = static_cast<const uint32_t*>(ws.dev[kDKs]);
+  a.items = static_cast<const ListItem*>(ws.dev[kDItems]);
+  a.n_items = L.n_items;
+  a.qperm = static_cast<const uint32_t*>(ws.dev[kDQperm]);
+  a.ids = static_cast<const uint32_t*>(ws.dev[kDIds]);
+  a.slot_off = static_cast<const uint64_t*>(ws.dev[kDSlot]);
+  a.L = L.kmax;
+  a.part = static_cast<uint64_t*>(ws.dev[kDPart]);
+  a.counter = ws.counter;
+  a.qthr = static_cast<uint64_t*>(ws.dev[kDThr]);
+  return a;
+}
+
+// Work items of the unfiltered queries i < rq with flags[i] != 0 (all of them when *all != 0),
+// in the layout the host planner gives one unfiltered group: chunk-major, qb queries per item.
+// The n_act items fill the END of the items array (max_items entries) and the work counter
+// starts at max_items - n_act, so the scan kernel's bound stays the capture-time max_items;
+// with no query chosen every block exits on its first fetch. qmap lists the chosen queries
+// for the merge, unused slots kNoneId.
+__global__ __launch_bounds__(1024) void lists_fallback_plan_kernel(
+    const uint32_t* __restrict__ flags, const uint32_t* __restrict__ all, uint32_t rq, uint32_t rows,
+    uint32_t C, uint32_t nch, uint32_t qb, uint32_t max_items, ListItem* __restrict__ items,
+    uint32_t* __restrict__ qperm, uint32_t* __restrict__ qmap, uint64_t* __restrict__ slot_off,
+    uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool every = *all != 0u;
+  const uint32_t per = (rq + 1023) / 1024;  // consecutive queries per thread (rq <= 4096)
+  const uint32_t q0 = min(rq, tid * per), q1 = min(rq, q0 + per);
+  uint32_t cnt = 0;
+  for (uint32_t q = q0; q < q1; ++q) cnt += (every || flags[q] != 0u) ? 1u : 0u;
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= uint32_t(off)) incl += y;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, nf = 0;
+  for (uint32_t w = 0; w < 16; ++w) {
+    before += w < wave ? wsum[w] : 0u;
+    nf += wsum[w];
+  }
+  uint32_t pos = before + incl - cnt;
+  for (uint32_t q = q0; q < q1; ++q) {
+    slot_off[q] = uint64_t(pos) * nch;
+    if (every || flags[q] != 0u) {
+      qperm[pos] = q;
+      qmap[pos] = q;
+      ++pos;
+    }
+    qthr[q] = ~uint64_t(0);
+  }
+  if (tid == 0) slot_off[rq] = uint64_t(nf) * nch;
+  for (uint32_t i = nf + tid; i < rq; i += 1024) qmap[i] = kNoneId;
+  const uint32_t nb = (nf + qb - 1) / qb, n_act = nch * nb;
+  for (uint32_t j = tid; j < n_act; j += 1024) {
+    const uint32_t c = j / nb, b = j - c * nb;
+    ListItem it;
+    it.lb = uint32_t(uint64_t(c) * C);
+    it.le = uint32_t(min(uint64_t(rows), uint64_t(c + 1) * C));
+    it.chunk = c;
+    it.qoff = b * qb;
+    it.nq = min(qb, nf - b * qb);
+    it.direct = 1;
+    items[max_items - n_act + j] = it;
+  }
+  if (tid == 0) *counter = max_items - n_act;
+}
+
 }  // namespace
 
 hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
@@ -694,24 +775,7 @@ hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (L.n_items) {
-    ListsArgs a;
-    a.grid = dim3(L.grid);
-    a.lds = L.lds;
-    a.st = st;
-    a.base = base;
-    a.dp = L.dp;
-    a.d = L.d;
-    a.q = d_q;
-    a.ks = static_cast<const uint32_t*>(ws.dev[kDKs]);
-    a.items = static_cast<const ListItem*>(ws.dev[kDItems]);
-    a.n_items = L.n_items;
-    a.qperm = static_cast<const uint32_t*>(ws.dev[kDQperm]);
-    a.ids = static_cast<const uint32_t*>(ws.dev[kDIds]);
-    a.slot_off = static_cast<const uint64_t*>(ws.dev[kDSlot]);
-    a.L = L.kmax;
-    a.part = static_cast<uint64_t*>(ws.dev[kDPart]);
-    a.counter = ws.counter;
-    a.qthr = static_cast<uint64_t*>(ws.dev[kDThr]);
+    const ListsArgs a = lists_args(ws, L, base, d_q, st);
     if (timing) timing->begin(st);
     if ((e = launch_lists_tile(L.qb, L.r, L.lpr, L.l2, L.kp, a)) != hipSuccess) return e;
     if (timing) timing->end(st, VSG_KERNEL_LISTS, 1);
@@ -744,6 +808,47 @@ int search_lists_host(ListsWorkspace& ws, const float* base, size_t n, size_t di
   std::memcpy(out_ids, ws.host[1], nq * kmax * sizeof(uint32_t));
   std::memcpy(out_dists, ws.host[2], nq * kmax * sizeof(float));
   return VSG_OK;
+}
+
+int prepare_lists_fallback(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,
+                           hipStream_t st, size_t nq, size_t k, std::string& err, ListsLaunch& out) {
+  if (nq == 0 || nq > 4096 || k == 0) {
+    err = "the exact fallback plans 1 to 4096 queries with k > 0";
+    return VSG_ERR_INVALID_ARGUMENT;
+  }
+  // every query chosen: the largest plan, whose buffers then fit every subset
+  const std::vector<uint64_t> range(2 * nq, VSG_ALL_ROWS);
+  const std::vector<uint32_t> ks(nq, uint32_t(k));
+  ListsPlanHost ph;
+  int rc = lists_prepare(ws, n, dim, dp, metric, num_cus, st, nullptr, nq, ks.data(), k, range.data(), nullptr, 0,
+                         err, ListsTiles{}, nullptr, ph, out);
+  if (rc) return rc;
+  out.rows = uint32_t(n);
+  out.nch = uint32_t((n + out.chunk - 1) / out.chunk);
+  if (uint64_t(out.nch) * ((nq + out.qb - 1) / out.qb) != out.n_items) {
+    err = "internal: fallback plan layout";
+    return VSG_ERR_DEVICE;
+  }
+  LISTS_HIP(dev_reserve(ws, kDQmap, nq * sizeof(uint32_t)));
+  LISTS_HIP(hipStreamSynchronize(st));  // the plan's host vectors are read by the uploads
+  return VSG_OK;
+}
+
+hipError_t lists_fallback_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* base, const float* d_q,
+                                  uint32_t rq, const uint32_t* flags, const uint32_t* all, uint32_t* d_ids,
+                                  float* d_dists, uint32_t ldo, hipStream_t st) {
+  if (rq == 0 || rq > L.nq) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lists_fallback_plan_kernel, dim3(1), dim3(1024), 0, st, flags, all, rq, L.rows,
+                     uint32_t(L.chunk), L.nch, uint32_t(L.qb), L.n_items, static_cast<ListItem*>(ws.dev[kDItems]),
+                     static_cast<uint32_t*>(ws.dev[kDQperm]), static_cast<uint32_t*>(ws.dev[kDQmap]),
+                     static_cast<uint64_t*>(ws.dev[kDSlot]), ws.counter, static_cast<uint64_t*>(ws.dev[kDThr]));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ListsArgs a = lists_args(ws, L, base, d_q, st);
+  if ((e = launch_lists_tile(L.qb, L.r, L.lpr, L.l2, L.kp, a)) != hipSuccess) return e;
+  return launch_merge(rq, static_cast<const uint64_t*>(ws.dev[kDPart]), static_cast<const uint64_t*>(ws.dev[kDSlot]),
+                      0, L.kmax, static_cast<const uint32_t*>(ws.dev[kDKs]), 0,
+                      static_cast<const uint32_t*>(ws.dev[kDQmap]), d_ids, d_dists, ldo, st);
 }
 
 int prepare_lists_device(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metric, int num_cus,

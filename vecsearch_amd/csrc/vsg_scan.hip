@@ -172,6 +172,7 @@ __global__ __launch_bounds__(256) void merge_lists_kernel(
   __shared__ uint64_t buf[kGather];
   __shared__ uint32_t cnt;
   const uint32_t q = qmap ? qmap[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
+  if (q == kNoneId) return;  // an unused output slot of a device-chosen query set
   const uint64_t l0 = list_off ? list_off[q] : uint64_t(q) * W;
   const uint32_t nl = list_off ? uint32_t(list_off[q + 1] - l0) : W;
   const uint32_t kq = min(min(ks ? ks[q] : k, ldo), L);
