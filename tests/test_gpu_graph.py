@@ -173,7 +173,7 @@ def _replay_and_check(gpu, x, make_q, nq, k, method, reps=3, metric="ip"):
 
 
 @pytest.mark.parametrize("nq,k,method", [(13, 10, "bf16"), (128, 10, "bf16"), (300, 100, "bf16"),
-                                         (128, 10, "fp8"), (64, 10, "auto")])
+                                         (128, 10, "fp8"), (100, 100, "fp8"), (64, 10, "auto")])
 def test_shortlist_graph_replays_match_eager(nq, k, method):
     x = dg.mixture(80_000, 384, seed=77)
     gpu = vs.GpuBruteForceIndex(384, "cosine")

@@ -305,15 +305,17 @@ struct Slot {
 };
 
 // The kk-th smallest (1-based, 1 <= kk <= c) of the high words of buf[0, c) (the order-preserving
-// bits of the estimates), by a 256-thread block: radix select, 8 bits a pass; wave 0 finds
+// bits of the estimates), by a block of NT >= 256 threads: radix select, 8 bits a pass; wave 0 finds
 // each pass's bin with a prefix sum over the 256 bins (4 a lane). Uses hist (256) and sel (2).
+template <int NT = 256>
 __device__ __forceinline__ uint32_t block_select_hi32(const uint64_t* buf, uint32_t c, uint32_t kk, uint32_t* hist, uint32_t* sel,
                                       uint32_t tid) {
+  static_assert(NT >= 256 && NT % 64 == 0, "a block of at least the 256 histogram bins");
   uint32_t prefix = 0, pmask = 0, rank = kk;
   for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[tid] = 0;
+    if (NT == 256 || tid < 256) hist[tid] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < c; i += 256) {
+    for (uint32_t i = tid; i < c; i += NT) {
       const uint32_t x = uint32_t(buf[i] >> 32);
       if ((x & pmask) == prefix) atomicAdd(&hist[(x >> shift) & 255], 1u);
     }

@@ -53,6 +53,9 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
   constexpr int kRowsPerStep = 64 / LPR;  // rows per wave instruction
   const int tid = threadIdx.x, lane = tid & 63, t = lane & (LPR - 1), g = lane / LPR;
   float* qw = lds + (tid >> 6) * QB * dp;  // this wave's queries
+  // a launch with no work left (a captured fallback with nothing to do) ends on a plain load
+  // instead of every wave's same-address atomic
+  if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items) return;
 
   for (;;) {
     uint32_t it = 0;
@@ -170,6 +173,7 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
   __shared__ uint32_t item_s;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, t = lane & 7, g = lane >> 3;
   const int o = t < 4 ? 2 * t : 22 - 2 * t;
+  if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items) return;  // no work
 
   for (;;) {
     __syncthreads();  // the previous item's LDS is no longer read

@@ -899,12 +899,14 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
 // candidates (small LDS, high occupancy), then CAP = kCandCap for the rest.
 constexpr int kCandCap = 8192;
 constexpr int kCandSmall = 2048;
+constexpr int kTopCap = 256;  // keys ranked after the rescore's select
 
-// Ascending bitonic sort of buf[0, n2) (n2 a power of two) by a 256-thread block.
+// Ascending bitonic sort of buf[0, n2) (n2 a power of two) by an NT-thread block.
+template <int NT>
 __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
   for (uint32_t size = 2; size <= n2; size <<= 1)
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = tid; i < n2 / 2; i += 256) {
+      for (uint32_t i = tid; i < n2 / 2; i += NT) {
         const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
         const bool up = (lo & size) == 0;
         const uint64_t a = buf[lo], b = buf[hi];
@@ -918,8 +920,10 @@ __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
 }
 
 // Handles the queries with cmin <= count <= CAP; with final_pass set, flags those above CAP.
-template <int D16, bool L2, int CAP>
-__global__ __launch_bounds__(256) void rescore_kernel(
+// NT threads per block: 1024 where the batch has few queries (one block each), so that a
+// query's candidate rows are read by 16 waves at once instead of 4.
+template <int D16, bool L2, int CAP, int NT>
+__global__ __launch_bounds__(NT) void rescore_kernel(
     const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ queries,
     const uint32_t* __restrict__ ks, uint32_t k, const float* __restrict__ E,
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ count,
@@ -929,8 +933,9 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   extern __shared__ uint64_t dyn[];
   uint64_t* buf = dyn;                                // CAP keys
   float* q_lds = reinterpret_cast<float*>(dyn + CAP);  // dp floats
-  __shared__ uint32_t nsel;
+  __shared__ uint32_t nsel, ntop;
   __shared__ uint32_t hist[256], sel[2];
+  __shared__ uint64_t top[kTopCap];
   const uint32_t q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
   const uint32_t kq = min(ks ? ks[q] : k, ldo);
   uint32_t* oi = out_ids + size_t(q) * ldo;
@@ -946,22 +951,22 @@ __global__ __launch_bounds__(256) void rescore_kernel(
     return;
   }
   if (tid == 0) overflow_flags[q] = 0;
-  for (uint32_t i = kq + tid; i < ldo; i += 256) {
+  for (uint32_t i = kq + tid; i < ldo; i += NT) {
     oi[i] = kNoneId;
     od[i] = __builtin_inff();
   }
-  for (uint32_t i = tid; i < dp; i += 256) q_lds[i] = i < d ? queries[size_t(q) * d + i] : 0.f;
+  for (uint32_t i = tid; i < dp; i += NT) q_lds[i] = i < d ? queries[size_t(q) * d + i] : 0.f;
   // Dk = the kk-th smallest estimate; keep dh <= Dk + 2E (compacted, in no particular order)
   const uint64_t* cq = cand + size_t(q) * kCandCap;
-  for (uint32_t i = tid; i < c; i += 256) buf[i] = cq[i];
+  for (uint32_t i = tid; i < c; i += NT) buf[i] = cq[i];
   if (tid == 0) nsel = 0;
   __syncthreads();
   const uint32_t kk = min(kq, c);
   if (kk > 0) {
-    const float Dk = unord_bits(block_select_hi32(buf, c, kk, hist, sel, tid));
+    const float Dk = unord_bits(block_select_hi32<NT>(buf, c, kk, hist, sel, tid));
     const float lim = round_up_f(double(Dk) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
     __syncthreads();  // every read of buf by the select is done
-    for (uint32_t i = tid; i < c; i += 256) {
+    for (uint32_t i = tid; i < c; i += NT) {
       const uint64_t v = cq[i];
       if (key_dist(v) <= lim) buf[atomicAdd(&nsel, 1u)] = v;
     }
@@ -969,9 +974,9 @@ __global__ __launch_bounds__(256) void rescore_kernel(
   __syncthreads();
   const uint32_t r = nsel;
   if (tid == 0) rescored[q] = r;
-  // exact distances, 16 rows per wave instruction group, 4 waves; a quad reads its
+  // exact distances, 16 rows per wave instruction group, NT / 64 waves; a quad reads its
   // candidate's row id and later overwrites that slot with the exact key
-  for (uint32_t base_i = (tid >> 6) * 16; base_i < r; base_i += 64) {
+  for (uint32_t base_i = (tid >> 6) * 16; base_i < r; base_i += 16 * (NT / 64)) {
     const uint32_t i = base_i + g;
     bool ok[1] = {i < r};
     const uint32_t row = key_id(buf[ok[0] ? i : 0]);
@@ -981,9 +986,39 @@ __global__ __launch_bounds__(256) void rescore_kernel(
     if (ok[0] && t == 0) buf[i] = make_key(dist[0][0], row);
   }
   __syncthreads();
-  // rank the exact keys (unique) and write the first kq
-  if (r <= 256) {
-    for (uint32_t i = tid; i < r; i += 256) {
+  // rank the exact keys (unique) and write the first kq: by counting for up to 256 keys; else
+  // through a radix select of the kq-th smallest distance bits T, which leaves the keys with
+  // distance bits <= T (every key of rank < kq, all others being larger) to be ranked among
+  // themselves; a bitonic sort only where ties put more than kTopCap keys at or below T
+  bool sorted = false;
+  if (r > 256 && kq > 0) {
+    const uint32_t th = block_select_hi32<NT>(buf, r, min(kq, r), hist, sel, tid);
+    if (tid == 0) ntop = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < r; i += NT) {
+      const uint64_t v = buf[i];
+      if (uint32_t(v >> 32) <= th) {
+        const uint32_t p = atomicAdd(&ntop, 1u);
+        if (p < uint32_t(kTopCap)) top[p] = v;
+      }
+    }
+    __syncthreads();
+    const uint32_t nt = ntop;
+    if (nt <= uint32_t(kTopCap)) {
+      for (uint32_t i = tid; i < nt; i += NT) {
+        const uint64_t v = top[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < nt; ++j) rank += top[j] < v;
+        if (rank < kq) {
+          oi[rank] = key_id(v);
+          od[rank] = key_dist(v);
+        }
+      }
+      sorted = true;
+    }
+  }
+  if (!sorted && r <= uint32_t(NT)) {
+    for (uint32_t i = tid; i < r; i += NT) {
       const uint64_t v = buf[i];
       uint32_t rank = 0;
       for (uint32_t j = 0; j < r; ++j) rank += buf[j] < v;
@@ -992,18 +1027,18 @@ __global__ __launch_bounds__(256) void rescore_kernel(
         od[rank] = key_dist(v);
       }
     }
-  } else {
-    uint32_t m2 = 512;
+  } else if (!sorted) {
+    uint32_t m2 = 2 * NT;  // <= CAP: r > NT candidates came from at most CAP
     while (m2 < r) m2 <<= 1;
-    for (uint32_t i = r + tid; i < m2; i += 256) buf[i] = kPadKey;
+    for (uint32_t i = r + tid; i < m2; i += NT) buf[i] = kPadKey;
     __syncthreads();
-    bitonic_sort_lds(buf, m2, tid);
-    for (uint32_t i = tid; i < kq && i < r; i += 256) {
+    bitonic_sort_lds<NT>(buf, m2, tid);
+    for (uint32_t i = tid; i < kq && i < r; i += NT) {
       oi[i] = key_id(buf[i]);
       od[i] = key_dist(buf[i]);
     }
   }
-  for (uint32_t i = r + tid; i < kq; i += 256) {
+  for (uint32_t i = r + tid; i < kq; i += NT) {
     oi[i] = kNoneId;
     od[i] = __builtin_inff();
   }
@@ -1208,16 +1243,22 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
   // every query's flag is set or cleared by the launch that handles it
   const size_t lds_small = size_t(kCandSmall) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
+  const bool wide = nq <= 256;  // fewer queries (blocks) than CUs: 16 waves per query
 #define VSG_RESCORE(D16, L2V)                                                                         \
   do {                                                                                                \
-    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall>), dim3(nq), dim3(256), lds_small, st,      \
-                       base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow, \
-                       rescored, 0u, 0);                                                              \
+    if (wide)                                                                                         \
+      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 1024>), dim3(nq), dim3(1024), lds_small, \
+                         st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,     \
+                         n_overflow, rescored, 0u, 0);                                                \
+    else                                                                                              \
+      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 256>), dim3(nq), dim3(256), lds_small,  \
+                         st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,     \
+                         n_overflow, rescored, 0u, 0);                                                \
     hipError_t e_ = hipFuncSetAttribute(                                                              \
-        reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap>),                           \
+        reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 256>),                      \
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                        \
     if (e_ != hipSuccess) return e_;                                                                  \
-    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap>), dim3(nq), dim3(256), lds, st, base, dp,   \
+    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap, 256>), dim3(nq), dim3(256), lds, st, base, dp, \
                        d, queries, ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow,          \
                        rescored, uint32_t(kCandSmall) + 1, 1);                                        \
   } while (0)
