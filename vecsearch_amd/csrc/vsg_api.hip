@@ -185,6 +185,7 @@ struct vsg_index {
   float* tiles = nullptr;  // the same rows in 16-row step-major tiles (streaming scans)
   int scan_waves_per_cu = 8;
   int auto_short_batch = 2;  // measured at 1M x 384: the shortlist is faster from 2 queries (and at 1)
+  int short_sample_stride = 0;  // pass 1 visits every stride-th tile (0: chosen per call)
   vsg::ListsTiles lists_tiles;  // filtered scan tile shape (0 = planner's choice or autotuner)
   bool lists_autotune = true;
   std::map<std::tuple<size_t, int, int, int>, vsg::ListsTiles> lists_tuned;  // by shape class
@@ -553,8 +554,10 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk)
     // (a looser bound only grows the candidate set, checked against its capacity): at least 4k
     // tiles and about 256 (one per CU, the pass costs one block's time), at most every 16th
-    const uint32_t tstride = uint32_t(std::max<size_t>(
-        1, std::min<size_t>({16, n_tiles / (4 * k), (size_t(n_tiles) + 255) / 256})));
+    const uint32_t tstride =
+        h->short_sample_stride > 0
+            ? uint32_t(std::max<size_t>(1, std::min<size_t>(size_t(h->short_sample_stride), n_tiles / std::max<size_t>(1, k))))
+            : uint32_t(std::max<size_t>(1, std::min<size_t>({16, n_tiles / (4 * k), (size_t(n_tiles) + 255) / 256})));
     const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
     a.tilemin = w.s_tilemin.p;
     a.emit = w.s_emit.p;
@@ -1029,6 +1032,7 @@ int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
   const std::string key(name);
   if (key == "scan_waves_per_cu") *value = h->scan_waves_per_cu;
   else if (key == "auto_short_batch") *value = h->auto_short_batch;
+  else if (key == "short_sample_stride") *value = h->short_sample_stride;
   else if (key == "lists_qb") *value = h->lists_tiles.qb;
   else if (key == "lists_r") *value = h->lists_tiles.r;
   else if (key == "lists_chunk") *value = int64_t(h->lists_tiles.chunk);
@@ -1054,6 +1058,11 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
   if (key == "scan_waves_per_cu") {
     if (value < 1 || value > 32) return fail(VSG_ERR_INVALID_ARGUMENT, "scan_waves_per_cu must be in 1..32");
     h->scan_waves_per_cu = int(value);
+    return VSG_OK;
+  }
+  if (key == "short_sample_stride") {
+    if (value < 0 || value > 256) return fail(VSG_ERR_INVALID_ARGUMENT, "short_sample_stride must be in 0..256");
+    h->short_sample_stride = int(value);
     return VSG_OK;
   }
   if (key == "auto_short_batch") {
