@@ -334,6 +334,22 @@ def test_shortlist_other_dims(d):
         assert_same(gpu.search(q, 10, method=method), oracle.search(x, q, 10, "ip", threads=8))
 
 
+@pytest.mark.parametrize("stride", [1, 7, 64])
+def test_shortlist_sample_stride(stride):
+    # any first-pass tile stride gives a valid (looser or tighter) threshold: the same results
+    x = dg.mixture(80_000, 384, seed=52)
+    q = dg.mixture(96, 384, seed=53)
+    gpu = vs.GpuBruteForceIndex(384, "cosine")
+    gpu.add(x)
+    gpu.set_param("short_sample_stride", stride)
+    assert gpu.get_param("short_sample_stride") == stride
+    exp = oracle.search(x, q, 10, "ip", threads=8)
+    for method in ("bf16", "fp8"):
+        assert_same(gpu.search(q, 10, method=method), exp)
+    with pytest.raises(ValueError):
+        gpu.set_param("short_sample_stride", 1000)
+
+
 @pytest.mark.parametrize("scale", [1e-3, 40.0])
 def test_shortlist_unnormalized_ip(scale):
     # inner products far from 1 in magnitude: the thresholds and score cuts must stay
