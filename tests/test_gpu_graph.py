@@ -220,3 +220,33 @@ def test_shortlist_region_overflow_eager_and_graph():
     np.testing.assert_array_equal(got[0], oi.astype(np.int64))
     np.testing.assert_array_equal(bits(got[1]), bits(od))
     _replay_and_check(gpu, x, make_q, 600, 10, "bf16", reps=2)
+
+
+def test_shortlist_graph_several_rounds():
+    # 4,100 queries: a full round of 4,096 and a ragged round of 4, each ending with the
+    # device-planned fallback sized for a full round
+    import torch
+
+    nq, k, d = 4100, 10, 128
+    x = dg.mixture(70_000, d, seed=78)
+    gpu = vs.GpuBruteForceIndex(d, "cosine")
+    gpu.add(x)
+    dev = torch.device("cuda:0")
+    q = dg.mixture(nq, d, seed=79)
+    d_q = torch.from_numpy(q).to(dev)
+    d_ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    d_dists = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    graph = vs.SearchGraph(gpu, d_q.data_ptr(), nq, k, d_ids.data_ptr(), d_dists.data_ptr(), method="bf16")
+    for _ in range(2):
+        graph.launch(0)
+        gpu.synchronize()
+        got_ids = d_ids.cpu().numpy().view(np.uint32)
+        got_d = d_dists.cpu().numpy()
+        ei, ed = gpu.search(q, k, method="bf16")
+        np.testing.assert_array_equal(np.where(got_ids == oracle.NONE, -1, got_ids.astype(np.int64)), ei)
+        np.testing.assert_array_equal(bits(got_d), bits(ed))
+    sel = np.r_[0:40, nq - 24:nq]  # both rounds against the oracle
+    oi, od = oracle.search(x, q[sel], k, "ip", threads=8)
+    np.testing.assert_array_equal(got_ids[sel], oi)
+    np.testing.assert_array_equal(bits(got_d[sel]), bits(od))
+    graph.close()
