@@ -1097,8 +1097,11 @@ hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
     geo = RegionGeometry{1u, n_regions, uint32_t(rows_frags(prec, a.KS) * 16), 6u};
   }
   if (n_regions == 0 || a.nq == 0) return hipErrorInvalidValue;  // count[] is written only below
-  // about 16 regions a slice (4 a wave), at most kScatterSlices slices
-  const uint32_t S = std::min(kScatterSlices, std::max(1u, (n_regions + 15) / 16));
+  // about 16 regions a slice (4 a wave), at most kScatterSlices slices, and at most 2^20 / nq so
+  // that the per-slice counters of a large batch (nq words each) stay well below its entries
+  // (at 4,096 queries: 256 slices, 4 MB of counters for about 18 MB of entries)
+  const uint32_t S = std::min({kScatterSlices, std::max(1u, (n_regions + 15) / 16),
+                               std::max(64u, (1u << 20) / a.nq)});
   const uint32_t per = (n_regions + S - 1) / S;
   const size_t lds = size_t(a.nq) * 4;
   hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap, geo,
