@@ -447,8 +447,14 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 // (ord(dh) << 32) | (query << kQsShift) | (row - first row of the block).
 constexpr int kQsWaves = 4;
 constexpr uint32_t kQsShift = 20;          // rows of a block < 2^20
-constexpr uint32_t kQsMaxRows = 4096;      // rows per block (FP8 scales of the block in LDS)
-constexpr uint32_t kQsMaxGroups = 4;       // query groups per launch (each reads the rows once)
+constexpr uint32_t kQsMaxRows = 4096;      // rows per FP8 block (the block's scales in LDS)
+// rows per block: FP8 by its LDS scales; BF16 by the 2^kQsShift row offsets of an entry
+__host__ __device__ constexpr uint32_t qs_max_rows(int prec) { return prec == 2 ? kQsMaxRows : 1u << 16; }
+// Query groups per launch (each reads the rows once): up to where the queries-stationary pass
+// beats the rows-stationary one, measured at 1M x 384 (pass 2, ms): BF16 4 groups (512
+// queries) 0.354 against 0.436 (with one wave of blocks: rows per block not capped at 4096);
+// FP8 3 groups 0.376 against 0.303 (its 4096-row cap leaves more blocks than slots).
+__host__ __device__ constexpr uint32_t qs_max_groups(int prec) { return prec == 1 ? 4u : 2u; }
 
 // row fragments per stage: the largest of 8, 4, 2, 1 whose stage fits 12 KB and splits into
 // whole 1 KB pieces, the same number for each wave; 1 if none fits (d = 768 in BF16: 24 KB).
@@ -1141,13 +1147,13 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
   if (ks != 2 && ks != 4 && ks != 8 && ks != 12 && ks != 24) return false;
   const uint32_t qpg = uint32_t(kQsWaves * qs_qw(prec, ks) * 16);  // queries per query group
   const uint32_t g = (nq + qpg - 1) / qpg;
-  if (nq == 0 || g > kQsMaxGroups || n == 0 || num_cus <= 0) return false;
+  if (nq == 0 || g > qs_max_groups(prec) || n == 0 || num_cus <= 0) return false;
   // blocks for all groups: about qs_bpc per CU (one pass over the grid), rows a multiple of
   // the stage's rows, row ranges a multiple of 8 (the XCD interleave of the groups)
   const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
   const uint32_t nb = std::max(8u, uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks)) / g);
   uint32_t r = (n + nb - 1) / nb;
-  r = std::min(kQsMaxRows, (r + unit - 1) / unit * unit);
+  r = std::min(qs_max_rows(prec), (r + unit - 1) / unit * unit);
   *rows = r;
   *blocks = ((n + r - 1) / r + 7) / 8 * 8;
   *groups = g;
@@ -1156,7 +1162,7 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
 }
 
 hipError_t launch_mfma_qs(int prec, const ShortScanArgs& a) {
-  if (a.nq == 0 || a.qs_blocks == 0 || a.qs_groups == 0 || a.qs_rows == 0 || a.qs_rows > kQsMaxRows ||
+  if (a.nq == 0 || a.qs_blocks == 0 || a.qs_groups == 0 || a.qs_rows == 0 || a.qs_rows > qs_max_rows(prec) ||
       a.qs_rows % (16u * uint32_t(qs_sf(prec, a.KS))) != 0)
     return hipErrorInvalidValue;
   return prec == 1 ? launch_qs_p<1>(a) : launch_qs_p<2>(a);
