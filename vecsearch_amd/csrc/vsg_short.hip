@@ -902,7 +902,8 @@ __global__ __launch_bounds__(256) void select_threshold_kernel(
 // Up to kCandCap candidates per query; a query with more is flagged and answered by the exact
 // scan. Candidates, then the exact keys, live in one dynamic LDS buffer of CAP keys followed by
 // the query (dp floats). Two launches: CAP = kCandSmall for every query with at most that many
-// candidates (small LDS, high occupancy), then CAP = kCandCap for the rest.
+// candidates (small LDS, high occupancy), then CAP = kCandCap for the rest; a batch of at most
+// 256 queries (one 16-wave block each, occupancy is not the limit) takes one CAP = kCandCap launch.
 constexpr int kCandCap = 8192;
 constexpr int kCandSmall = 2048;
 constexpr int kTopCap = 256;  // keys ranked after the rescore's select
@@ -1252,17 +1253,22 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
   // every query's flag is set or cleared by the launch that handles it
   const size_t lds_small = size_t(kCandSmall) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
-  const bool wide = nq <= 256;  // fewer queries (blocks) than CUs: 16 waves per query
+  const bool wide = nq <= 256;  // fewer queries (blocks) than CUs: 16 waves per query, one launch
 #define VSG_RESCORE(D16, L2V)                                                                         \
   do {                                                                                                \
-    if (wide)                                                                                         \
-      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 1024>), dim3(nq), dim3(1024), lds_small, \
-                         st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,     \
-                         n_overflow, rescored, 0u, 0);                                                \
-    else                                                                                              \
-      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 256>), dim3(nq), dim3(256), lds_small,  \
-                         st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,     \
-                         n_overflow, rescored, 0u, 0);                                                \
+    if (wide) { /* one block per query either way: one launch with the full buffer */               \
+      hipError_t w_ = hipFuncSetAttribute(                                                            \
+          reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 1024>),                   \
+          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                      \
+      if (w_ != hipSuccess) return w_;                                                                \
+      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap, 1024>), dim3(nq), dim3(1024), lds, st,    \
+                         base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,         \
+                         n_overflow, rescored, 0u, 1);                                                \
+      break;                                                                                          \
+    }                                                                                                 \
+    hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 256>), dim3(nq), dim3(256), lds_small,    \
+                       st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,       \
+                       n_overflow, rescored, 0u, 0);                                                  \
     hipError_t e_ = hipFuncSetAttribute(                                                              \
         reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 256>),                      \
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                        \
