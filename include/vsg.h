@@ -113,7 +113,8 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
  *   "scan_waves_per_cu"    waves per CU of the streaming scan (1..32, default 8);
  *   "auto_short_batch"     smallest batch that AUTO sends to the shortlist (default 2);
  *   "short_sample_stride"  the shortlist's first pass visits every stride-th tile of rows
- *                          (0 = chosen per call, default; kept to at least k tiles);
+ *                          (0 = chosen per call from the batch size and the tile count,
+ *                          default; kept to at least k tiles);
  *   "lists_qb", "lists_r", "lists_chunk", "lists_blocks_per_cu", "lists_lanes_per_row"
  *                          a fixed tile of the filtered scan (queries per work item 1/2/4/8/16,
  *                          row groups per lane step 2/4, allowed rows per item, blocks per CU,
