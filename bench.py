@@ -68,6 +68,7 @@ def parse():
                     help="rows of the serving-loop base (0 to skip the serving measurement)")
     ap.add_argument("--serve-dim", type=int, default=768)
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="offered-load time per rate")
+    ap.add_argument("--serve-slots", type=int, default=2, help="worker streams of the serving loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB megabytes (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings), then exit")
@@ -155,13 +156,14 @@ def serving(args, log):
     # warm-up: builds the BF16 copy and tunes the filtered-scan tiles of the common batch shapes;
     # the measured server then never pauses a batch for tuning (classes it has not seen run the
     # planner's tile)
-    with vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=2, method="auto") as warm_srv:
+    with vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=args.serve_slots, method="auto") as warm_srv:
         for r in [make_request() for _ in range(16)]:
             warm_srv.wait(warm_srv.submit(*r))
         for _ in range(3):
             for t in [warm_srv.submit(*r) for r in [make_request() for _ in range(100)]]:
                 warm_srv.wait(t)
-    srv = vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=2, method="auto", tune=False)
+    srv = vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=args.serve_slots, method="auto",
+                    tune=False)
 
     def run(reqs, rate):
         """Submit reqs at Poisson times (rate requests/s; 0 = all at once); returns latencies."""
@@ -217,7 +219,8 @@ def serving(args, log):
     srv.close()
     return {"workload": f"{n:,} x {d} fp32 cosine base + BF16 copy resident; requests of 1-512 queries "
                         f"(log-uniform), k in {{10, 100}}, 50% with one Zipf(s=1.1) group filter of 100",
-            "server": {"slots": 2, "max_batch_queries": 4096, "workspace_bytes": 8 << 30, "tune": False},
+            "server": {"slots": args.serve_slots, "max_batch_queries": 4096, "workspace_bytes": 8 << 30,
+                       "tune": False},
             "saturation_requests_per_s": round(sat_rps, 1), "saturation_qps": round(sat_qps, 1),
             "rates": rows, "admission_waits": st["admission_waits"],
             "peak_inflight_bytes": st["peak_inflight_bytes"], "max_batch_requests": st["max_batch_requests"],
