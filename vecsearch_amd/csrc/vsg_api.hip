@@ -551,18 +551,19 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     a.qfrag = w.s_qfrag.p;
     a.qaux = w.s_qaux.p;
     a.nq = rq;
-    // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk)
-    // (a looser bound only grows the candidate set, checked against its capacity): at least 4k
-    // tiles and about 256 (one per CU, the pass costs one block's time), at most every 16th
-    // (up to 256 queries a pass-1 block is short and the looser bound costs little: every 16th
-    // tile, measured 1.5-2 % faster at batch 128 than one tile per CU, which wins at 4,096)
+    // pass 1 visits every tstride-th tile (a subset's k-th smallest minimum still bounds Dk;
+    // a looser bound only grows the candidate set, checked against its capacity): at least 4k
+    // tiles, at most every 16th. The query stages of a visited tile are shared by qsplit blocks
+    // so that the pass, one block's time, fills about one block per CU: at 1M rows 123 tiles x 2
+    // shares, where one tile per CU (245 x 1) was 2 % slower at batch 4,096 (its tile ran every
+    // query) and 1.5-2.5 % slower at batches 8-256 (a tighter bound that saved less).
     const uint32_t tstride =
         h->short_sample_stride > 0
             ? uint32_t(std::max<size_t>(1, std::min<size_t>(size_t(h->short_sample_stride), n_tiles / std::max<size_t>(1, k))))
-        : rq <= 256
-            ? uint32_t(std::max<size_t>(1, std::min<size_t>(16, n_tiles / (4 * k))))
-            : uint32_t(std::max<size_t>(1, std::min<size_t>({16, n_tiles / (4 * k), (size_t(n_tiles) + 255) / 256})));
+            : uint32_t(std::max<size_t>(1, std::min<size_t>(16, n_tiles / (4 * k))));
     const uint32_t space1 = (n_tiles + tstride - 1) / tstride;
+    const uint32_t nst1 = uint32_t(((rq + 15) / 16 + size_t(vsg::short_group(p, ks)) - 1) / size_t(vsg::short_group(p, ks)));
+    const uint32_t qsplit = std::max(1u, std::min(nst1, uint32_t(h->num_cus) / space1));
     a.tilemin = w.s_tilemin.p;
     a.emit = w.s_emit.p;
     a.wave_cap = wave_cap;
@@ -575,7 +576,9 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     {
       TimedScope ts(h, st, VSG_KERNEL_SAMPLE);
       a.tstride = tstride;
-      a.n_space = a.n_blocks = space1;
+      a.n_space = space1;
+      a.qsplit = qsplit;
+      a.n_blocks = space1 * qsplit;
       VSG_HIP(vsg::launch_mfma_scan(p, 1, a));
       VSG_HIP(vsg::launch_select_threshold(st, w.s_tilemin.p, space1, rq, uint32_t(qpad), nullptr, uint32_t(k),
                                            w.s_E.p, w.s_qscale.p, w.s_thr.p, w.s_qaux.p));
@@ -584,6 +587,7 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     {
       TimedScope ts(h, st, VSG_KERNEL_MFMA);
       a.tstride = 1;
+      a.qsplit = 1;
       a.n_space = a.n_blocks = n_tiles;
       if (qs) {
         a.qs_blocks = qs_blocks;

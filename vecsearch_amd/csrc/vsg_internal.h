@@ -42,6 +42,7 @@ struct ShortScanArgs {
   uint32_t n_blocks;  // one block per visited tile
   uint32_t n_space;   // visited tiles: pass 1 ceil(n_tiles / tstride), pass 2 n_tiles
   uint32_t tstride;   // pass 1 visits every tstride-th tile; pass 2 uses 1
+  uint32_t qsplit = 1;  // pass 1: blocks per visited tile (n_blocks = n_space * qsplit)
   uint32_t* tilemin;
   uint64_t* emit;         // pass 2: per wave region (8 per tile), wave_cap entries
   uint32_t wave_cap;

@@ -266,11 +266,18 @@ template <int PREC, int KS, int PASS>
 __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t tstride,
-    uint32_t n_space, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ emit, uint32_t wave_cap,
-    uint32_t* __restrict__ emit_count) {
+    uint32_t n_space, uint32_t qsplit, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ emit,
+    uint32_t wave_cap, uint32_t* __restrict__ emit_count) {
   typedef typename Frag<PREC>::T FT;
   typedef StageShape<PREC, KS> SH;
   constexpr int RF = SH::RF, G = SH::G, LOADS = SH::LOADS;
+  // qsplit blocks share a visited tile (pass 1 of a large batch), each taking a contiguous share
+  // of the query stages [st0, nstages); pass 2 runs with qsplit = 1
+  const uint32_t nst_all = (((nq + 15) >> 4) + G - 1) / G;
+  const uint32_t per_part = (nst_all + qsplit - 1) / qsplit;
+  const uint32_t st0 = min(nst_all, (blockIdx.x % qsplit) * per_part);
+  const uint32_t nstages = min(nst_all, st0 + per_part);
+  if (st0 == nstages) return;  // no stage left for this share (block-uniform, before any barrier)
   // Four ring slots as distinct LDS objects: the compiler can then tell the slot being read
   // from the slots still being filled by LDS-DMA loads, and waits only on the counted vmcnt.
   __shared__ __attribute__((aligned(16))) uint8_t slot0[SH::STAGE_BYTES];
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   __shared__ __attribute__((aligned(16))) uint8_t slot3[SH::STAGE_BYTES];
   __shared__ float red[2][kWaves * G * 16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint32_t j = blockIdx.x;  // visited tile
+  const uint32_t j = blockIdx.x / qsplit;  // visited tile
   const uint32_t tile = j * tstride;
   const uint32_t nrb = (n + 15) >> 4;
   const uint32_t rb0 = tile * (kWaves * RF) + wave * RF;  // first row block of this wave
@@ -306,7 +313,6 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 
   // The caller pads the query fragments (and qaux) to whole stages, so every stage load is
   // in bounds and every wave issues exactly LOADS loads per stage.
-  const uint32_t nstages = (((nq + 15) >> 4) + G - 1) / G;
   const uint8_t* qsrc = static_cast<const uint8_t*>(qfrag);
   auto slot = [&](auto I) -> uint8_t* {
     constexpr int v = decltype(I)::value & 3;
@@ -340,7 +346,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   auto step = [&](auto I, uint32_t st) {
     wait_vm<(kAhead - 1) * LOADS>();  // the two later stages may stay in flight
     ring_barrier();  // every wave's part of stage st is in LDS; stage st - 1 is consumed
-    if (PASS == 1 && st > 0) flush_min(st - 1);
+    if (PASS == 1 && st > st0) flush_min(st - 1);
     fetch(slot(Slot<decltype(I)::value + kAhead>()), min(st + kAhead, nstages - 1));
     const uint8_t* stage = slot(I);
     const FT* as = reinterpret_cast<const FT*>(stage);
@@ -413,11 +419,11 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     }
   };
 
-  // nstages >= 1 (the launch has at least one query)
-  fetch(slot0, 0);
-  fetch(slot1, min(1u, nstages - 1));
-  fetch(slot2, min(2u, nstages - 1));
-  for (uint32_t st = 0;;) {  // leaves the loop right after the last stage
+  // nstages > st0 (checked above)
+  fetch(slot0, st0);
+  fetch(slot1, min(st0 + 1, nstages - 1));
+  fetch(slot2, min(st0 + 2, nstages - 1));
+  for (uint32_t st = st0;;) {  // leaves the loop right after the last stage
     step(Slot<0>(), st);
     if (++st == nstages) break;
     step(Slot<1>(), st);
@@ -1183,11 +1189,11 @@ hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
   const dim3 grid(a.n_blocks), block(64 * kWaves);
   if (pass == 1)
     hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.emit, a.wave_cap,
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.qsplit, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   else
     hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.tilemin, a.emit, a.wave_cap,
+                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, 1u, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   return hipGetLastError();
 }
@@ -1220,6 +1226,8 @@ int short_tile_rows_for(int prec, int ks) { return kWaves * rows_frags(prec, ks)
 
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a) {
   if (a.nq == 0 || a.n_blocks == 0) return hipErrorInvalidValue;  // a block streams at least one stage
+  if (pass == 1 && (a.qsplit == 0 || a.n_blocks != a.n_space * a.qsplit)) return hipErrorInvalidValue;
+  if (pass == 2 && a.n_blocks != a.n_space) return hipErrorInvalidValue;
   return prec == 1 ? launch_mfma_p<1>(pass, a) : launch_mfma_p<2>(pass, a);
 }
 
