@@ -59,8 +59,8 @@ def test_graph_limits():
     d_q = torch.zeros((64, d), dtype=torch.float32, device=dev)
     d_ids = torch.empty((64, 10), dtype=torch.int32, device=dev)
     d_dists = torch.empty((64, 10), dtype=torch.float32, device=dev)
-    with pytest.raises(ValueError):  # the legacy tiled method is not capturable
-        vs.SearchGraph(gpu, d_q.data_ptr(), 64, 10, d_ids.data_ptr(), d_dists.data_ptr(), method="tiled")
+    with pytest.raises(ValueError):  # k > 128
+        vs.SearchGraph(gpu, d_q.data_ptr(), 64, 200, d_ids.data_ptr(), d_dists.data_ptr(), method="skinny")
     g = vs.SearchGraph(gpu, d_q.data_ptr(), 64, 10, d_ids.data_ptr(), d_dists.data_ptr(), method="skinny")
     g.launch(0)
     gpu.synchronize()
@@ -173,7 +173,7 @@ def _replay_and_check(gpu, x, make_q, nq, k, method, reps=3, metric="ip"):
 
 
 @pytest.mark.parametrize("nq,k,method", [(13, 10, "bf16"), (128, 10, "bf16"), (300, 100, "bf16"),
-                                         (128, 10, "fp8"), (100, 100, "fp8"), (64, 10, "auto")])
+                                         (128, 10, "fp8"), (100, 100, "fp8"), (64, 10, "auto"), (40, 10, "tiled")])
 def test_shortlist_graph_replays_match_eager(nq, k, method):
     x = dg.mixture(80_000, 384, seed=77)
     gpu = vs.GpuBruteForceIndex(384, "cosine")

@@ -437,8 +437,8 @@ PYBIND11_MODULE(_vecsearch, m) {
            py::arg("index"), py::arg("d_queries"), py::arg("nq"), py::arg("k"), py::arg("d_ids"),
            py::arg("d_dists"), py::arg("method") = "skinny",
            "Capture one search of nq device-resident queries into a HIP graph (vsg_graph_create): the\n"
-           "streaming scan, or the certified shortlist (bf16 / fp8 / auto from auto_short_batch queries)\n"
-           "with its exact fallback planned on the device; method 'tiled' is refused.")
+           "streaming scan, or the certified shortlist (tiled / bf16 / fp8 / auto from auto_short_batch\n"
+           "queries) with its exact fallback planned on the device: the path the eager search takes.")
       .def_static(
           "lists",
           [](py::object index, uintptr_t d_q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,

@@ -1539,13 +1539,11 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
     return fail(VSG_ERR_INVALID_ARGUMENT, "a captured search needs nq > 0, k > 0 and device buffers");
   std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
-  if (method == VSG_METHOD_TILED)
-    return fail(VSG_ERR_INVALID_ARGUMENT, "the tiled method cannot be captured (use AUTO, NAIVE, SKINNY, BF16 or FP8)");
   if (h->n == 0) return fail(VSG_ERR_INVALID_ARGUMENT, "cannot capture a search of an empty index");
   // the path an eager call takes (enqueue_search): the shortlist (p > 0) or the streaming scan
   int p = 0;
   if (short_supported(h, k)) {
-    if (method == VSG_METHOD_BF16) p = 1;
+    if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16) p = 1;
     else if (method == VSG_METHOD_FP8) p = 2;
     else if (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch)) p = 1;
   }
