@@ -179,12 +179,11 @@ int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t
  * own workspace, on the path an eager call of the same method, nq and k takes: the streaming
  * scan and its merge (NAIVE / SKINNY, and AUTO below auto_short_batch) or the certified
  * shortlist (TILED / BF16 / FP8, and AUTO from auto_short_batch queries). Inside a captured
- * shortlist,
- * queries whose candidates overflow (or every query of a round whose emission regions
- * overflow) are answered by an exact fallback that the device plans itself: a one-block kernel
- * picks them and one persistent launch of the filtered scan plus its merge rewrites their
- * rows, so the replay waits on nothing. vsg_graph_launch replays the graph
- * on `stream`, reading whatever d_queries holds at that point and writing d_ids / d_dists.
+ * shortlist, queries whose candidates overflow (or every query of a round whose emission
+ * regions overflow) are answered by an exact fallback that the device plans itself: a one-block
+ * kernel picks them and one persistent launch of the filtered scan plus its merge rewrites
+ * their rows, so the replay waits on nothing. vsg_graph_launch replays the graph on `stream`,
+ * reading whatever d_queries holds at that point and writing d_ids / d_dists.
  * The buffers and the index must outlive the graph. The graph is bound to the base's size and
  * address at capture: after vsg_add, a launch returns INVALID (capture a new graph). */
 typedef struct vsg_graph vsg_graph;
