@@ -453,9 +453,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 // (ord(dh) << 32) | (query << kQsShift) | (row - first row of the block).
 constexpr int kQsWaves = 4;
 constexpr uint32_t kQsShift = 20;          // rows of a block < 2^20
-constexpr uint32_t kQsMaxRows = 4096;      // rows per FP8 block (the block's scales in LDS)
-// rows per block: FP8 by its LDS scales; BF16 by the 2^kQsShift row offsets of an entry
-__host__ __device__ constexpr uint32_t qs_max_rows(int prec) { return prec == 2 ? kQsMaxRows : 1u << 16; }
+constexpr uint32_t kQsMaxRows = 1u << 16;  // rows per block (entries carry 2^kQsShift row offsets)
 // Query groups per launch (each reads the rows once): up to where the queries-stationary pass
 // beats the rows-stationary one, measured at 1M x 384 (pass 2, ms): BF16 4 groups (512
 // queries) 0.354 against 0.436 (with one wave of blocks: rows per block not capped at 4096);
@@ -479,11 +477,15 @@ __host__ __device__ constexpr int qs_hs(int prec, int ks) { return ks * (prec ==
 __host__ __device__ constexpr int qs_stage(int prec, int ks) {
   return qs_sf(prec, ks) * ks * (prec == 1 ? 1024 : 512) / qs_hs(prec, ks);
 }
-__host__ __device__ constexpr int qs_lds(int prec, int ks) {  // the ring and, for FP8, the row scales
-  return 4 * qs_stage(prec, ks) + (prec == 2 ? int(kQsMaxRows) * 4 : 0);
+__host__ __device__ constexpr int qs_lds(int prec, int ks) {  // the ring (FP8: with its scales)
+  return 4 * (qs_stage(prec, ks) + (prec == 2 ? kQsWaves * 256 * ((qs_sf(prec, ks) * 16 + 63) / 64) : 0));
 }
 __host__ __device__ constexpr int qs_bpc(int prec, int ks) {  // blocks per CU
-  return (160 << 10) / qs_lds(prec, ks) > 3 ? 3 : (160 << 10) / qs_lds(prec, ks) < 1 ? 1 : (160 << 10) / qs_lds(prec, ks);
+  // (FP8 at d = 768 keeps 2: its 172 registers a lane exceed the 168 of three blocks a CU)
+  return prec == 2 && ks > 12 ? 2
+         : (160 << 10) / qs_lds(prec, ks) > 3 ? 3
+         : (160 << 10) / qs_lds(prec, ks) < 1 ? 1
+                                               : (160 << 10) / qs_lds(prec, ks);
 }
 // query fragments per wave: A operands of all k-steps in at most 96 registers
 __host__ __device__ constexpr int qs_qw(int prec, int ks) { return 2 * ks * (prec == 1 ? 4 : 2) <= 96 ? 2 : 1; }
@@ -515,11 +517,13 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
   constexpr int FB = KS * SH::CB;  // bytes of one row fragment (all k-steps)
   // distinct LDS objects per slot: the compiler then waits only on the counted vmcnt
   static_assert(NBUF == 4, "four ring slots");
-  __shared__ __attribute__((aligned(16))) uint8_t s0[ST];
-  __shared__ __attribute__((aligned(16))) uint8_t s1[ST];
-  __shared__ __attribute__((aligned(16))) uint8_t s2[ST];
-  __shared__ __attribute__((aligned(16))) uint8_t s3[ST];
-  __shared__ float xs_lds[PREC == 2 ? kQsMaxRows : 1];
+  // FP8: each slot also holds the row scales of its stage (XN 64-row wave loads), one copy per wave
+  constexpr int XN = PREC == 2 ? (SF * 16 + 63) / 64 : 0, XB = kQsWaves * 256 * XN;
+  static_assert(PREC != 2 || HS == 1, "FP8 stages hold whole fragments");
+  __shared__ __attribute__((aligned(16))) uint8_t s0[ST + XB];
+  __shared__ __attribute__((aligned(16))) uint8_t s1[ST + XB];
+  __shared__ __attribute__((aligned(16))) uint8_t s2[ST + XB];
+  __shared__ __attribute__((aligned(16))) uint8_t s3[ST + XB];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Block (bx, qg) of nbx x ngroups: the query groups of one row range are 8 workgroups apart,
   // so they run on the same XCD (workgroups go round-robin over the 8 XCDs) at about the same
@@ -553,14 +557,21 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       qsc[j][i] = (PREC == 2 && qok) ? qaux[qf * 32 + 16 + q4 + i] : 1.f;
     }
   }
-  if (PREC == 2)
-    for (uint32_t i = threadIdx.x; i < rows_per_block; i += 64 * kQsWaves)
-      xs_lds[i] = row0 + i < n ? xscale[row0 + i] : 1.f;
   wait_vm<0>();
   __syncthreads();
 
   const uint8_t* xsrc = static_cast<const uint8_t*>(xfrag);
-  auto fetch = [&](uint8_t* dst, uint32_t st) __attribute__((always_inline)) {
+  // FP8: the row scales of a stage's SF fragments travel with the stage by LDS-DMA (lane l:
+  // row 16 rf + l of the stage's first fragment rf), each wave into its own copy, so the
+  // stage's counted wait covers them and no register load crosses the loop; VL = vector-memory
+  // loads per thread per stage
+  constexpr int VL = LOADS + XN;
+  auto slot = [&](auto I) -> uint8_t* {
+    constexpr int v = decltype(I)::value % NBUF;
+    return v == 0 ? s0 : v == 1 ? s1 : v == 2 ? s2 : s3;
+  };
+  auto fetch = [&](auto I, uint32_t st) __attribute__((always_inline)) {
+    uint8_t* dst = slot(I);
     const uint32_t fg = st / HS, part = st - fg * HS;
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
@@ -570,13 +581,18 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       if (rf >= rf1) rf = rf0;  // past the block's rows: a fragment that is never read
       load_lds16(xsrc + size_t(rf) * FB + (at - fi * FB) + lane * 16, dst + piece * 1024);
     }
+#pragma unroll
+    for (int r = 0; r < XN; ++r) {  // rows past the stage's SF * 16: loaded, never read
+      const uint32_t row = (rf0 + fg * SF) * 16 + uint32_t(r * 64 + lane);
+      load_lds4(xscale + (row < n ? row : n - 1), dst + ST + (wave * XN + r) * 256);
+    }
   };
   uint32_t n_emit = 0;  // wave-uniform
   f32x4m_t acc[QW];     // carried from part 0 to part 1 of a fragment when HS = 2
   // part: st % HS, known at compile time (the loop runs 8 steps a trip from a multiple of 8)
-  auto compute = [&](const uint8_t* stage, uint32_t st, auto part_c) __attribute__((always_inline)) {
+  auto compute = [&](auto I, uint32_t st, auto part_c) __attribute__((always_inline)) {
     constexpr int part = decltype(part_c)::value;
-    const FT* bs = reinterpret_cast<const FT*>(stage);
+    const FT* bs = reinterpret_cast<const FT*>(slot(I));
 #pragma unroll
     for (int fi = 0; fi < SF; ++fi) {
       const uint32_t rf = rf0 + st / HS * SF + fi;  // (past the block's rows: masked, not skipped)
@@ -594,7 +610,9 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       // value i of acc[j] belongs to query (fragment of j) * 16 + q4 + i and row rf * 16 + lane % 16
       const uint32_t row = rf * 16 + (lane & 15);
       const bool ok = rf < rf1 && row < n;
-      const float xs = (PREC == 2 && ok) ? xs_lds[row - row0] : 1.f;
+      float xs = 1.f;
+      if constexpr (PREC == 2)
+        xs = ok ? reinterpret_cast<const float*>(slot(I) + ST + wave * XN * 256)[fi * 16 + (lane & 15)] : 1.f;
       float sc[QW][4];
       uint64_t any = 0;
 #pragma unroll
@@ -623,21 +641,17 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       }
     }
   };
-  auto slot = [&](auto I) -> uint8_t* {
-    constexpr int v = decltype(I)::value % NBUF;
-    return v == 0 ? s0 : v == 1 ? s1 : v == 2 ? s2 : s3;
-  };
   // consume stage st from slot I; refill slot I + NBUF - 1 (consumed at st - 1) with stage
   // st + NBUF - 1 (past the last stage: the last stage again, into a slot nobody reads)
   auto step = [&](auto I, uint32_t st) __attribute__((always_inline)) {
-    wait_vm<(NBUF - 2) * LOADS>();  // the later stages may stay in flight
+    wait_vm<(NBUF - 2) * VL>();  // the later stages may stay in flight
     ring_barrier();
-    fetch(slot(Slot<decltype(I)::value + NBUF - 1>()), min(st + NBUF - 1, nst - 1));
-    compute(slot(I), st, Slot<decltype(I)::value % HS>());
+    fetch(Slot<decltype(I)::value + NBUF - 1>(), min(st + NBUF - 1, nst - 1));
+    compute(I, st, Slot<decltype(I)::value % HS>());
   };
-  fetch(s0, 0);
-  fetch(s1, min(1u, nst - 1));
-  fetch(s2, min(2u, nst - 1));
+  fetch(Slot<0>(), 0);
+  fetch(Slot<1>(), min(1u, nst - 1));
+  fetch(Slot<2>(), min(2u, nst - 1));
   // The loop runs eight steps a trip: at the loop head the compiler waits for every load in
   // flight (it does not carry its per-slot tracking of the LDS-DMA loads around the loop).
   for (uint32_t st = 0;;) {  // leaves the loop right after the last stage
@@ -1160,7 +1174,7 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
   const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
   const uint32_t nb = std::max(8u, uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks)) / g);
   uint32_t r = (n + nb - 1) / nb;
-  r = std::min(qs_max_rows(prec), (r + unit - 1) / unit * unit);
+  r = std::min(kQsMaxRows, (r + unit - 1) / unit * unit);
   *rows = r;
   *blocks = ((n + r - 1) / r + 7) / 8 * 8;
   *groups = g;
@@ -1169,7 +1183,7 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
 }
 
 hipError_t launch_mfma_qs(int prec, const ShortScanArgs& a) {
-  if (a.nq == 0 || a.qs_blocks == 0 || a.qs_groups == 0 || a.qs_rows == 0 || a.qs_rows > qs_max_rows(prec) ||
+  if (a.nq == 0 || a.qs_blocks == 0 || a.qs_groups == 0 || a.qs_rows == 0 || a.qs_rows > kQsMaxRows ||
       a.qs_rows % (16u * uint32_t(qs_sf(prec, a.KS))) != 0)
     return hipErrorInvalidValue;
   return prec == 1 ? launch_qs_p<1>(a) : launch_qs_p<2>(a);
