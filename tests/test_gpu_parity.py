@@ -266,7 +266,7 @@ def test_mixture_100k_384(batch):
 @pytest.fixture(scope="module")
 def mixture_70k():
     x = dg.mixture(70_000, 384, seed=40)
-    q = dg.mixture(300, 384, seed=41)
+    q = dg.mixture(600, 384, seed=41)  # above 512 queries: the rows-stationary pass 2
     gpu = vs.GpuBruteForceIndex(384, "cosine")
     gpu.add(x)
     return x, q, gpu
@@ -291,6 +291,19 @@ def test_shortlist_small_batches(mixture_70k, method, nq, k):
     got = gpu.search(q[:nq], k, method=method)
     assert_same(got, oracle.search(x, q[:nq], k, "ip", threads=8))
     big = gpu.search(q, k, method=method)
+    np.testing.assert_array_equal(got[0], big[0][:nq])
+    np.testing.assert_array_equal(bits(got[1]), bits(big[1][:nq]))
+
+
+@pytest.mark.parametrize("method", ["bf16", "fp8"])
+@pytest.mark.parametrize("nq", [384, 512])
+def test_shortlist_three_and_four_query_groups(mixture_70k, method, nq):
+    # 3 and 4 query groups of the queries-stationary pass 2 (FP8: each stage's row scales in
+    # its ring slot): the oracle's bits, and the same as inside the 600-query rows-stationary batch
+    x, q, gpu = mixture_70k
+    got = gpu.search(q[:nq], 10, method=method)
+    assert_same(got, oracle.search(x, q[:nq], 10, "ip", threads=8))
+    big = gpu.search(q, 10, method=method)
     np.testing.assert_array_equal(got[0], big[0][:nq])
     np.testing.assert_array_equal(bits(got[1]), bits(big[1][:nq]))
 

@@ -455,10 +455,10 @@ constexpr int kQsWaves = 4;
 constexpr uint32_t kQsShift = 20;          // rows of a block < 2^20
 constexpr uint32_t kQsMaxRows = 1u << 16;  // rows per block (entries carry 2^kQsShift row offsets)
 // Query groups per launch (each reads the rows once): up to where the queries-stationary pass
-// beats the rows-stationary one, measured at 1M x 384 (pass 2, ms): BF16 4 groups (512
-// queries) 0.354 against 0.436 (with one wave of blocks: rows per block not capped at 4096);
-// FP8 3 groups 0.376 against 0.303 (its 4096-row cap leaves more blocks than slots).
-__host__ __device__ constexpr uint32_t qs_max_groups(int prec) { return prec == 1 ? 4u : 2u; }
+// beats the rows-stationary one, measured at 1M x 384 (pass 2 at 512 queries: BF16 0.354 ms
+// against 0.436, FP8 0.304 against 0.383), given one wave of blocks (rows per block up to
+// kQsMaxRows) and, for FP8, the row scales carried by the stages.
+constexpr uint32_t kQsMaxGroups = 4;
 
 // row fragments per stage: the largest of 8, 4, 2, 1 whose stage fits 12 KB and splits into
 // whole 1 KB pieces, the same number for each wave; 1 if none fits (d = 768 in BF16: 24 KB).
@@ -1168,7 +1168,7 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
   if (ks != 2 && ks != 4 && ks != 8 && ks != 12 && ks != 24) return false;
   const uint32_t qpg = uint32_t(kQsWaves * qs_qw(prec, ks) * 16);  // queries per query group
   const uint32_t g = (nq + qpg - 1) / qpg;
-  if (nq == 0 || g > qs_max_groups(prec) || n == 0 || num_cus <= 0) return false;
+  if (nq == 0 || g > kQsMaxGroups || n == 0 || num_cus <= 0) return false;
   // blocks for all groups: about qs_bpc per CU (one pass over the grid), rows a multiple of
   // the stage's rows, row ranges a multiple of 8 (the XCD interleave of the groups)
   const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
