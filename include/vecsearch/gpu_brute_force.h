@@ -15,10 +15,18 @@
 
 struct vsg_index;
 
+// Metric is the reference's own type (distance.h:15) wherever the reference headers are on the
+// include path, so this header compiles beside brute_force.h / hnsw.h (as the reference's
+// python/bindings.cpp:10-14 includes them); standalone it declares the same enum.
+#if __has_include("vecsearch/distance.h")
+#include "vecsearch/distance.h"
+#else
 namespace vecsearch {
-
-// distance.h:15
 enum class Metric : uint32_t { L2 = 0, InnerProduct = 1 };
+}  // namespace vecsearch
+#endif
+
+namespace vecsearch {
 
 // gpu_brute_force.h:16-21. Every method returns the same ids and distances; they differ in
 // how the scan is run: Naive = one query per streaming pass, Skinny = up to 8 queries per

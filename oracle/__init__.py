@@ -38,7 +38,7 @@ def build():
     """Compile the C oracle (and oracle/_ref when the reference tree is present)."""
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir(os.environ.get("VS_REFERENCE", "/root/reference")):
-        subprocess.run(["make", "-s", "-C", HERE, "ref",
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "dropin",
                         "REF=" + os.environ.get("VS_REFERENCE", "/root/reference")], check=True)
 
 

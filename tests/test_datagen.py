@@ -28,3 +28,14 @@ def test_zipf_groups_and_ranges():
     r = dg.query_ranges(qg, off)
     assert r.shape == (64, 2) and set(np.unique(ks)) <= {10, 100}
     assert np.all(r[:, 1] - r[:, 0] == np.bincount(g, minlength=100)[qg])
+
+
+def test_filtered_groups_span():
+    """The filtered workload's allowed sets run from about 12 % to about 0.01 % of the rows."""
+    p = dg.zipf_weights(dg.F_GROUPS, dg.F_S, dg.F_SHIFT, dg.F_CUTOFF)
+    assert abs(p[0] - 0.12) < 1e-5 and abs(p[-1] - 1e-4) < 1e-8 and np.all(np.diff(p) < 0)
+    # with no shift and no cutoff the law is plain Zipf(1.1): 23.4 % .. 0.15 % over 100 ranks
+    z = dg.zipf_weights(100, 1.1)
+    assert abs(z[0] - 0.2338) < 1e-3 and abs(z[-1] - 0.001475) < 1e-5
+    frac = np.bincount(dg.filtered_groups(892_800), minlength=100) / 892_800
+    assert 0.11 < frac.max() < 0.13 and 0.00004 < frac.min() < 0.0002

@@ -1,6 +1,6 @@
 """The certified shortlist's error bound rests on the accuracy of its two MFMA instructions.
 
-tests/gpu_probes/mfma_accuracy (built with the library) runs chains of 1-24 instructions of
+tests/gpu_probes/mfma_accuracy (built by tests/gpu_probes/build.py) runs chains of 1-24 instructions of
 v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x32_fp8_fp8 on seeded random operands and
 reports, per instruction and chain, the largest |mfma - exact| / sum |products|. The bound
 takes 2 gamma_d |q^| |x^| for the accumulation (BF16: within the sequential fp32 model) and,

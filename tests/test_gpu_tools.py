@@ -13,13 +13,21 @@ pytestmark = pytest.mark.gpu
 
 def test_gpu_bench_csv(tmp_path):
     out = tmp_path / "sweep.csv"
-    gpu_bench.main(["--n", "30000", "--dim", "64", "--batches", "1,8,64", "--backends", "hip,hip-naive,hip-skinny",
+    gpu_bench.main(["--n", "30000", "--dim", "64", "--batches", "1,8,64", "--methods", "auto,naive,skinny",
                     "--min-time", "0.05", "--check", "32", "--out", str(out)])
     rows = list(csv.DictReader(open(out)))
+    # the columns of the reference's results/gpu/*.csv (bench/gpu_bench.py:121)
     assert list(rows[0].keys()) == ["backend", "device", "n", "dim", "metric", "k", "batch", "ms_per_batch",
                                     "qps", "recall_vs_cpu"]
     assert len(rows) == 9 and {r["backend"] for r in rows} == {"hip", "hip-naive", "hip-skinny"}
     assert all(float(r["recall_vs_cpu"]) == 1.0 and float(r["qps"]) > 0 for r in rows)
+
+
+def test_median_call_seconds_rule():
+    """One untimed call, then at least three timed calls and at least min_time seconds."""
+    calls = []
+    gpu_bench.median_call_seconds(lambda: calls.append(1), 0.0)
+    assert len(calls) == 4
 
 
 def test_ground_truth_file(tmp_path):

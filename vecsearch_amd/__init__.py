@@ -5,8 +5,9 @@ python/bindings.cpp:151-197):
 
     GpuBruteForceIndex(dim, metric="l2", device=0)
         .add(x)                                    # (n, dim) float32 (other dtypes are cast)
-        .search(queries, k=10, filter=None, *, method="auto")
+        .search(queries, k=10, method="auto", filter=None)
             -> (ids int64[nq, k], dists float32[nq, k]), nearest first, missing = -1 / inf
+            (a mask given as the third argument is the filter: search(queries, k, filter))
         .search_batch(queries, ks, ranges, allow_ids)      # per-query k and allowed ids
     cuda_available(), hip_available(), normalize(x), recall_at_k(found, truth, k)
 

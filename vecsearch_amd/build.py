@@ -77,24 +77,10 @@ def build_ext():
     return EXT
 
 
-PROBE_SRC = os.path.join(ROOT, "tests", "gpu_probes", "mfma_accuracy.hip")
-PROBE = os.path.join(ROOT, "tests", "gpu_probes", "mfma_accuracy")
-
-
-def build_probe():
-    """The MFMA accuracy probe the GPU tests run (a program, not part of the library)."""
-    hipcc = os.path.join(ROCM, "bin", "hipcc")
-    tmp = PROBE + ".tmp"
-    _run([hipcc, *HIP_FLAGS, "-o", tmp, PROBE_SRC])
-    os.replace(tmp, PROBE)
-    return PROBE
-
-
 def build():
     build_hip()
     build_ext()
-    build_probe()
-    return LIB, EXT, PROBE
+    return LIB, EXT
 
 
 if __name__ == "__main__":

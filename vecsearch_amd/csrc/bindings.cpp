@@ -5,8 +5,9 @@
 // become (1, d) (:174), check_2d -> ValueError (:44-47), mask parsing as_mask (:28-36),
 // parse_metric (:38-42), ids as int64 with -1 for missing (:49-56), the GIL released around
 // add/search (:161, :179-182), method names auto/naive/skinny/tiled (:168-173).
-// Added: search(..., filter=None) with the HNSW binding's mask convention (:84-91), and
-// search_batch for per-query k and per-query allowed-id lists.
+// Added: search(queries, k, method, filter=None) with the HNSW binding's mask convention
+// (:84-91), where a mask given third is the filter, and search_batch for per-query k and
+// per-query allowed-id lists.
 #include <algorithm>
 #include <cstdint>
 #include <stdexcept>
@@ -237,13 +238,24 @@ PYBIND11_MODULE(_vecsearch, m) {
           py::arg("x"))
       .def(
           "search",
-          [](GpuBruteForceIndex& self, FloatArray q, size_t k, const py::object& filter,
-             const std::string& method) {
+          [](GpuBruteForceIndex& self, FloatArray q, size_t k, const py::object& method_or_filter,
+             const py::object& filter) {
+            // The reference's third parameter is the method name (bindings.cpp:166-185); a mask
+            // in that position is the filter, as in search(queries, k, filter).
+            std::string method = "auto";
+            py::object flt = filter;
+            if (py::isinstance<py::str>(method_or_filter)) {
+              method = method_or_filter.cast<std::string>();
+            } else if (!method_or_filter.is_none()) {
+              if (!filter.is_none())
+                throw std::invalid_argument("search: a filter was given twice (third argument and filter=)");
+              flt = method_or_filter;
+            }
             const GpuMethod gm = parse_method(method);
             if (q.ndim() == 1) q = q.reshape({py::ssize_t(1), q.shape(0)});
             check_2d(q, self.dim());
             const size_t nq = q.shape(0);
-            Mask mask = as_mask(filter, self.size());
+            Mask mask = as_mask(flt, self.size());
             py::array_t<uint32_t> ids({nq, k});
             py::array_t<float> dists({nq, k});
             {
@@ -255,10 +267,11 @@ PYBIND11_MODULE(_vecsearch, m) {
             }
             return py::make_tuple(ids_to_int64(ids), dists);
           },
-          py::arg("queries"), py::arg("k") = 10, py::arg("filter") = py::none(), py::kw_only(),
-          py::arg("method") = "auto",
+          py::arg("queries"), py::arg("k") = 10, py::arg("method") = "auto", py::arg("filter") = py::none(),
           "Exact top-k on the GPU, k <= 128. Returns (ids[int64, nq x k], distances[float32, nq x k]).\n"
-          "filter: optional bool/uint8 mask over ids (len(self) entries); only allowed ids are returned.")
+          "method: 'auto', 'naive', 'skinny', 'tiled', 'bf16' or 'fp8' (the same results).\n"
+          "filter: optional bool/uint8 mask over ids (len(self) entries); only allowed ids are returned.\n"
+          "A mask passed as the third argument is taken as the filter: search(queries, k, filter).")
       .def(
           "search_batch",
           [](GpuBruteForceIndex& self, FloatArray q, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> ks,

@@ -111,9 +111,9 @@ constexpr size_t kPartBudgetBytes = size_t(256) << 20;  // per-round candidate l
 
 namespace {
 
-// Per-search scratch: device workspace, pinned staging and statistics. The index owns one
-// for its host API (calls serialised by the index mutex); each serving slot owns another, so
-// batches in flight on different streams never share scratch memory.
+// Per-search scratch: device workspace, pinned staging and statistics. Each eager call leases
+// one (a Lane, below) for its duration; each serving slot and each captured graph owns another,
+// so searches in flight on different streams never share scratch memory.
 struct Workspace {
   // shortlist
   DevBuf<uint8_t> s_qfrag;
@@ -173,6 +173,19 @@ struct Workspace {
   }
 };
 
+// One eager search in flight: a stream, a workspace, and an event marking the end of the last
+// work that used the workspace, on whichever stream it was enqueued (vsg_search_device runs on
+// the caller's stream). The next user of the lane makes its own stream wait for that event
+// before touching the workspace, so two searches never overlap on the same scratch memory.
+struct Lane {
+  hipStream_t st = nullptr;
+  Workspace w;
+  hipEvent_t done = nullptr;
+  bool pending = false;  // `done` was recorded and the next user must wait for it
+};
+
+constexpr size_t kStageBytes = size_t(64) << 20;  // one pinned upload slice
+
 }  // namespace
 
 struct vsg_index {
@@ -200,14 +213,33 @@ struct vsg_index {
   } prec[3];
   int ks = 0;  // 32-dim k-steps of the copies (0: shortlist unsupported for this dim)
 
-  hipStream_t stream = nullptr;
-  Workspace ws;  // the host API's workspace
+  hipStream_t stream = nullptr;  // adds, low-precision copies, and vsg_search_device(NULL)
 
-  // Locking: `mu` serialises calls of the host API (as the reference's per-index mutex does,
-  // gpu_brute_force.cu:351). Searches hold `rw` shared and writers (add, reserve, set_param)
-  // hold it exclusively, so serving slots search concurrently with the host API but never
-  // with a change of the base. prec_mu guards the lazy low-precision copies, tune_mu the tile
-  // table, timing_mu the event list.
+  // Eager searches lease a Lane each (at most max_lanes at once; a caller beyond that waits for
+  // one to come back), so concurrent callers of the host API run side by side on their own
+  // streams instead of queueing on one lock as the reference's per-index mutex makes them do
+  // (gpu_brute_force.cu:351,430).
+  std::mutex lanes_mu;
+  std::condition_variable lanes_cv;
+  std::vector<std::unique_ptr<Lane>> lanes;
+  std::vector<Lane*> idle;
+  int max_lanes = 4;
+  int lanes_busy = 0, lanes_peak = 0;
+
+  // What vsg_short_stats and the lists_last_* parameters report: the last call's figures.
+  std::mutex last_mu;
+  uint64_t last_emitted = 0, last_rescored = 0, last_fallback = 0;
+  vsg::ListsTiles last_tiles;
+
+  // pinned upload slices (vsg_add, vsg_add_hnsw_file): one is filled while the other's copy runs
+  float* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_busy[2] = {false, false};
+
+  // Locking: searches hold `rw` shared and writers (add, reserve, set_param) hold it
+  // exclusively, so searches run concurrently with each other but never with a change of the
+  // base. `mu` orders the writers and the parameter / timing calls among themselves; prec_mu
+  // guards the lazy low-precision copies, tune_mu the tile table, timing_mu the event list.
   std::mutex mu;
   std::shared_mutex rw;
   std::mutex prec_mu, tune_mu, timing_mu;
@@ -227,6 +259,19 @@ struct vsg_index {
       (void)hipEventDestroy(t.b);
     }
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& l : lanes) {
+      if (l->st) (void)hipStreamSynchronize(l->st);
+      if (l->done) (void)hipEventSynchronize(l->done);
+    }
+    for (auto& l : lanes) {
+      l->w.release();
+      if (l->done) (void)hipEventDestroy(l->done);
+      if (l->st) (void)hipStreamDestroy(l->st);
+    }
+    for (int i = 0; i < 2; ++i) {
+      if (stage[i]) (void)hipHostFree(stage[i]);
+      if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
+    }
     if (base) (void)hipFree(base);
     if (tiles) (void)hipFree(tiles);
     for (auto& p : prec) {
@@ -235,7 +280,6 @@ struct vsg_index {
       if (p.norms) (void)hipFree(p.norms);
       if (p.maxes) (void)hipFree(p.maxes);
     }
-    ws.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -273,6 +317,110 @@ int grow(vsg_index* h, size_t need) {
   h->cap = cap;
   return VSG_OK;
 }
+
+// Appends m host rows to the base (the caller holds rw exclusively and has grown the base).
+// fill(dst, r0, c) writes rows r0 .. r0 + c of the source into dst, packed (c x dim floats);
+// they go through two pinned 64 MB slices: while one slice is copied to HBM on the index
+// stream, the next is filled on the host. Rows are padded from dim to dp floats in the slice.
+template <class Fill>
+int upload_rows(vsg_index* h, size_t m, Fill&& fill) {
+  const size_t row_floats = h->dp;
+  const size_t per = std::max<size_t>(1, kStageBytes / (row_floats * sizeof(float)));
+  for (int i = 0; i < 2; ++i) {
+    if (!h->stage[i])
+      VSG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->stage[i]), per * row_floats * sizeof(float),
+                            hipHostMallocDefault));
+    if (!h->stage_ev[i]) VSG_HIP(hipEventCreateWithFlags(&h->stage_ev[i], hipEventDisableTiming));
+  }
+  int s = 0;
+  for (size_t r0 = 0; r0 < m; r0 += per, s ^= 1) {
+    const size_t c = std::min(per, m - r0);
+    if (h->stage_busy[s]) {
+      VSG_HIP(hipEventSynchronize(h->stage_ev[s]));
+      h->stage_busy[s] = false;
+    }
+    float* buf = h->stage[s];
+    int rc = fill(buf, r0, c);
+    if (rc) return rc;
+    if (h->dim != row_floats) {  // spread the packed rows out to dp floats, last row first
+      for (size_t i = c; i-- > 0;) {
+        std::memmove(buf + i * row_floats, buf + i * h->dim, h->dim * sizeof(float));
+        std::memset(buf + i * row_floats + h->dim, 0, (row_floats - h->dim) * sizeof(float));
+      }
+    }
+    VSG_HIP(hipMemcpyAsync(h->base + (h->n + r0) * row_floats, buf, c * row_floats * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+    VSG_HIP(hipEventRecord(h->stage_ev[s], h->stream));
+    h->stage_busy[s] = true;
+  }
+  VSG_HIP(vsg::launch_tile_rows(h->stream, h->base, h->tiles, uint32_t(h->n), uint32_t(h->n + m),
+                                uint32_t(h->dp)));
+  VSG_HIP(hipStreamSynchronize(h->stream));
+  h->stage_busy[0] = h->stage_busy[1] = false;
+  h->n += m;
+  return VSG_OK;
+}
+
+// Set while this thread records a HIP graph: no timing events go into a capture.
+thread_local bool t_capturing = false;
+
+bool timing_on(const vsg_index* h) { return h->timing && !t_capturing; }
+
+// A lane held for one eager search. acquire() takes an idle lane (or makes a new one, up to
+// max_lanes, or waits) and makes `st` wait for the lane's previous work; the destructor marks
+// the lane's workspace busy until `st` reaches the end of this search's work and returns it.
+class LaneLease {
+ public:
+  explicit LaneLease(vsg_index* h) : h_(h) {}
+  LaneLease(const LaneLease&) = delete;
+  LaneLease& operator=(const LaneLease&) = delete;
+  ~LaneLease() {
+    if (!lane_) return;
+    if (st_ && hipEventRecord(lane_->done, st_) == hipSuccess) lane_->pending = true;
+    std::lock_guard<std::mutex> g(h_->lanes_mu);
+    h_->idle.push_back(lane_);
+    h_->lanes_busy--;
+    h_->lanes_cv.notify_one();
+  }
+  // st == nullptr: the lane's own stream.
+  int acquire(hipStream_t st = nullptr) {
+    {
+      std::unique_lock<std::mutex> lk(h_->lanes_mu);
+      for (;;) {
+        if (!h_->idle.empty()) {
+          lane_ = h_->idle.back();
+          h_->idle.pop_back();
+          break;
+        }
+        if (int(h_->lanes.size()) < h_->max_lanes) {
+          auto l = std::make_unique<Lane>();
+          hipError_t e = hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking);
+          if (e == hipSuccess) e = hipEventCreateWithFlags(&l->done, hipEventDisableTiming);
+          if (e != hipSuccess) {
+            if (l->st) (void)hipStreamDestroy(l->st);
+            return fail_hip("search lane", e);
+          }
+          lane_ = l.get();
+          h_->lanes.push_back(std::move(l));
+          break;
+        }
+        h_->lanes_cv.wait(lk);
+      }
+      h_->lanes_busy++;
+      h_->lanes_peak = std::max(h_->lanes_peak, h_->lanes_busy);
+    }
+    st_ = st ? st : lane_->st;
+    if (lane_->pending) VSG_HIP(hipStreamWaitEvent(st_, lane_->done, 0));
+    return VSG_OK;
+  }
+  Workspace& w() { return lane_->w; }
+  hipStream_t stream() const { return st_; }
+
+ private:
+  vsg_index* h_;
+  Lane* lane_ = nullptr;
+  hipStream_t st_ = nullptr;
+};
 
 // Plan of one streaming scan over the base: n_blocks workgroups of 4 waves, each wave owning
 // rows_per_wave consecutive rows (a multiple of 64); each block leaves one k-list per query
@@ -317,7 +465,7 @@ struct TimedScope {
   uint32_t launches = 0;
   hipEvent_t a = nullptr, b = nullptr;
   TimedScope(vsg_index* h_, hipStream_t st_, int kind_) : h(h_), st(st_), kind(kind_) {
-    if (!h->timing) return;
+    if (!timing_on(h)) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
         hipEventRecord(a, st) != hipSuccess) {
       h->timing = false;  // give up timing rather than fail the search
@@ -325,7 +473,7 @@ struct TimedScope {
   }
   ~TimedScope() {
     if (!a || !b) return;
-    if (h->timing && hipEventRecord(b, st) == hipSuccess) {
+    if (timing_on(h) && hipEventRecord(b, st) == hipSuccess) {
       std::lock_guard<std::mutex> g(h->timing_mu);
       h->timed.push_back({a, b, kind, launches});
     } else {
@@ -342,7 +490,7 @@ struct IndexTiming : vsg::TimingSink {
   explicit IndexTiming(vsg_index* h_) : h(h_) {}
   void begin(hipStream_t st) override {
     a = nullptr;
-    if (!h->timing) return;
+    if (!timing_on(h)) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventRecord(a, st) != hipSuccess) {
       if (a) (void)hipEventDestroy(a);
       a = nullptr;
@@ -352,7 +500,7 @@ struct IndexTiming : vsg::TimingSink {
   void end(hipStream_t st, int kind, uint32_t launches) override {
     if (!a) return;
     hipEvent_t b = nullptr;
-    if (h->timing && hipEventCreate(&b) == hipSuccess && hipEventRecord(b, st) == hipSuccess) {
+    if (timing_on(h) && hipEventCreate(&b) == hipSuccess && hipEventRecord(b, st) == hipSuccess) {
       std::lock_guard<std::mutex> g(h->timing_mu);
       h->timed.push_back({a, b, kind, launches});
     } else {
@@ -626,7 +774,7 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
       continue;
     }
     const uint32_t nover = w.s_hover.p[0];
-    if (h->timing || nover) {
+    if (timing_on(h) || nover) {
       counts.resize(rq);
       rescored.resize(rq);
       VSG_HIP(hipMemcpy(counts.data(), w.s_count.p, rq * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -680,9 +828,24 @@ int enqueue_search(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size
     if (method == VSG_METHOD_TILED || method == VSG_METHOD_BF16) p = 1;
     else if (method == VSG_METHOD_FP8) p = 2;
     else if (method == VSG_METHOD_AUTO && nq >= size_t(h->auto_short_batch)) p = 1;
-    if (p) return enqueue_short(h, w, d_q, nq, k, p, d_ids, d_dists, st);
+    if (p) {
+      const int rc = enqueue_short(h, w, d_q, nq, k, p, d_ids, d_dists, st);
+      if (rc == VSG_OK) {
+        std::lock_guard<std::mutex> g(h->last_mu);
+        h->last_emitted = w.st_emitted;
+        h->last_rescored = w.st_rescored;
+        h->last_fallback = w.st_fallback;
+      }
+      return rc;
+    }
   }
   return enqueue_exact(h, w, d_q, nq, k, method, d_mask, d_ids, d_dists, st);
+}
+
+// The tile of the last filtered call, for the lists_last_* parameters.
+void note_lists(vsg_index* h, const Workspace& w) {
+  std::lock_guard<std::mutex> g(h->last_mu);
+  h->last_tiles = w.lists_last;
 }
 
 void fill_missing(uint32_t* ids, float* dists, size_t count) {
@@ -929,13 +1092,10 @@ int vsg_add(vsg_index* h, const float* data, size_t m) {
   VSG_HIP(hipSetDevice(h->device));
   int rc = grow(h, h->n + m);
   if (rc) return rc;
-  VSG_HIP(hipMemcpy2DAsync(h->base + h->n * h->dp, h->dp * sizeof(float), data, h->dim * sizeof(float),
-                           h->dim * sizeof(float), m, hipMemcpyHostToDevice, h->stream));
-  VSG_HIP(vsg::launch_tile_rows(h->stream, h->base, h->tiles, uint32_t(h->n), uint32_t(h->n + m),
-                                uint32_t(h->dp)));
-  VSG_HIP(hipStreamSynchronize(h->stream));
-  h->n += m;
-  return VSG_OK;
+  return upload_rows(h, m, [&](float* dst, size_t r0, size_t c) {
+    std::memcpy(dst, data + r0 * h->dim, c * h->dim * sizeof(float));
+    return VSG_OK;
+  });
 }
 
 int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int method,
@@ -944,38 +1104,43 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
   if (rc) return rc;
   if (nq == 0 || k == 0) return VSG_OK;
   if (!queries || !out_ids || !out_dists) return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
-  std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
-  Workspace& w = h->ws;
   if (h->n == 0) {
     fill_missing(out_ids, out_dists, nq * k);
     return VSG_OK;
   }
   VSG_HIP(hipSetDevice(h->device));
+  size_t allowed = h->n;
   if (mask) {
-    const size_t allowed = size_t(std::count_if(mask, mask + h->n, [](uint8_t v) { return v != 0; }));
+    allowed = size_t(std::count_if(mask, mask + h->n, [](uint8_t v) { return v != 0; }));
     if (allowed == 0) {
       fill_missing(out_ids, out_dists, nq * k);  // hnsw.cpp:326: no allowed row, no result
       return VSG_OK;
     }
-    if (allowed * 4 < h->n) {
-      // Sparse filter: gather the allowed rows (the exact_below regime) instead of streaming
-      // the whole base; every query shares the one id list.
-      std::vector<uint32_t> ids;
-      ids.reserve(allowed);
-      for (size_t i = 0; i < h->n; ++i)
-        if (mask[i]) ids.push_back(uint32_t(i));
-      std::vector<uint64_t> range(2 * nq);
-      for (size_t i = 0; i < nq; ++i) {
-        range[2 * i] = 0;
-        range[2 * i + 1] = allowed;
-      }
-      std::vector<uint32_t> ks(nq, uint32_t(k));
-      IndexTiming tm(h);
-      return vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus,
-                                    h->stream, queries, nq, ks.data(), k, range.data(), ids.data(),
-                                    ids.size(), out_ids, out_dists, g_err, &tm, h->lists_tiles, &w.lists_last);
+  }
+  LaneLease lease(h);
+  if ((rc = lease.acquire())) return rc;
+  Workspace& w = lease.w();
+  const hipStream_t st = lease.stream();
+  if (mask && allowed * 4 < h->n) {
+    // Sparse filter: gather the allowed rows (the exact_below regime) instead of streaming
+    // the whole base; every query shares the one id list.
+    std::vector<uint32_t> ids;
+    ids.reserve(allowed);
+    for (size_t i = 0; i < h->n; ++i)
+      if (mask[i]) ids.push_back(uint32_t(i));
+    std::vector<uint64_t> range(2 * nq);
+    for (size_t i = 0; i < nq; ++i) {
+      range[2 * i] = 0;
+      range[2 * i + 1] = allowed;
     }
+    std::vector<uint32_t> ks(nq, uint32_t(k));
+    IndexTiming tm(h);
+    rc = vsg::search_lists_host(w.lists, h->base, h->n, h->dim, h->dp, h->metric, h->num_cus, st, queries,
+                                nq, ks.data(), k, range.data(), ids.data(), ids.size(), out_ids, out_dists,
+                                g_err, &tm, h->lists_tiles, &w.lists_last);
+    if (rc == VSG_OK) note_lists(h, w);
+    return rc;
   }
   hipError_t e;
   if ((e = w.q.reserve(nq * h->dim)) != hipSuccess) return fail_hip("workspace", e);
@@ -987,16 +1152,16 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
   const uint8_t* d_mask = nullptr;
   if (mask) {
     if ((e = w.mask.reserve(h->n)) != hipSuccess) return fail_hip("workspace", e);
-    VSG_HIP(hipMemcpyAsync(w.mask.p, mask, h->n, hipMemcpyHostToDevice, h->stream));
+    VSG_HIP(hipMemcpyAsync(w.mask.p, mask, h->n, hipMemcpyHostToDevice, st));
     d_mask = w.mask.p;
   }
   std::memcpy(w.hq.p, queries, nq * h->dim * sizeof(float));
-  VSG_HIP(hipMemcpyAsync(w.q.p, w.hq.p, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  rc = enqueue_search(h, w, w.q.p, nq, k, method, d_mask, w.ids.p, w.dists.p, h->stream);
+  VSG_HIP(hipMemcpyAsync(w.q.p, w.hq.p, nq * h->dim * sizeof(float), hipMemcpyHostToDevice, st));
+  rc = enqueue_search(h, w, w.q.p, nq, k, method, d_mask, w.ids.p, w.dists.p, st);
   if (rc) return rc;
-  VSG_HIP(hipMemcpyAsync(w.hids.p, w.ids.p, nq * k * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-  VSG_HIP(hipMemcpyAsync(w.hdists.p, w.dists.p, nq * k * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  VSG_HIP(hipStreamSynchronize(h->stream));
+  VSG_HIP(hipMemcpyAsync(w.hids.p, w.ids.p, nq * k * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  VSG_HIP(hipMemcpyAsync(w.hdists.p, w.dists.p, nq * k * sizeof(float), hipMemcpyDeviceToHost, st));
+  VSG_HIP(hipStreamSynchronize(st));
   std::memcpy(out_ids, w.hids.p, nq * k * sizeof(uint32_t));
   std::memcpy(out_dists, w.hdists.p, nq * k * sizeof(float));
   return VSG_OK;
@@ -1008,7 +1173,6 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
   if (rc) return rc;
   if (nq == 0 || k == 0) return VSG_OK;
   if (!d_queries || !d_ids || !d_dists) return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
-  std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
   VSG_HIP(hipSetDevice(h->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : h->stream;
@@ -1016,7 +1180,11 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
     VSG_HIP(vsg::launch_fill_missing(st, d_ids, d_dists, nq * k));
     return VSG_OK;
   }
-  return enqueue_search(h, h->ws, d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
+  // The lane's workspace is ordered on `st` both ways: `st` waits for the lane's previous
+  // work, and the lane is marked busy until `st` has finished this search.
+  LaneLease lease(h);
+  if ((rc = lease.acquire(st))) return rc;
+  return enqueue_search(h, lease.w(), d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
 int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
@@ -1028,10 +1196,15 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
   if (nq == 0 || kmax == 0) return VSG_OK;
   if (!queries || !ks || !range || !out_ids || !out_dists)
     return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
-  std::lock_guard<std::mutex> g(h->mu);
   std::shared_lock<std::shared_mutex> rl(h->rw);
   VSG_HIP(hipSetDevice(h->device));
-  return run_lists(h, h->ws, h->stream, queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids, out_dists);
+  LaneLease lease(h);
+  int rc = lease.acquire();
+  if (rc) return rc;
+  rc = run_lists(h, lease.w(), lease.stream(), queries, nq, ks, kmax, range, allow_ids, n_ids, out_ids,
+                 out_dists);
+  if (rc == VSG_OK) note_lists(h, lease.w());
+  return rc;
 }
 
 int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
@@ -1049,11 +1222,17 @@ int vsg_get_param(vsg_index* h, const char* name, int64_t* value) {
   else if (key == "lists_tuned_classes") {
     std::lock_guard<std::mutex> t(h->tune_mu);
     *value = int64_t(h->lists_tuned.size());
-  } else if (key == "lists_last_qb") *value = h->ws.lists_last.qb;
-  else if (key == "lists_last_r") *value = h->ws.lists_last.r;
-  else if (key == "lists_last_chunk") *value = int64_t(h->ws.lists_last.chunk);
-  else if (key == "lists_last_lanes_per_row") *value = h->ws.lists_last.lanes_per_row;
-  else if (key == "lists_lanes_per_row") *value = h->lists_tiles.lanes_per_row;
+  } else if (key.rfind("lists_last_", 0) == 0 || key == "search_lanes_peak" || key == "search_lanes") {
+    std::lock_guard<std::mutex> l(h->last_mu);
+    std::lock_guard<std::mutex> p(h->lanes_mu);
+    if (key == "lists_last_qb") *value = h->last_tiles.qb;
+    else if (key == "lists_last_r") *value = h->last_tiles.r;
+    else if (key == "lists_last_chunk") *value = int64_t(h->last_tiles.chunk);
+    else if (key == "lists_last_lanes_per_row") *value = h->last_tiles.lanes_per_row;
+    else if (key == "search_lanes_peak") *value = h->lanes_peak;
+    else if (key == "search_lanes") *value = h->max_lanes;
+    else return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
+  } else if (key == "lists_lanes_per_row") *value = h->lists_tiles.lanes_per_row;
   else return fail(VSG_ERR_INVALID_ARGUMENT, "unknown parameter: " + key);
   return VSG_OK;
 }
@@ -1158,10 +1337,10 @@ int vsg_kernel_times(vsg_index* h, double* ms_out, uint64_t* launches_out) {
 
 int vsg_short_stats(vsg_index* h, uint64_t* emitted, uint64_t* rescored, uint64_t* fallbacks) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
-  std::lock_guard<std::mutex> g(h->mu);
-  if (emitted) *emitted = h->ws.st_emitted;
-  if (rescored) *rescored = h->ws.st_rescored;
-  if (fallbacks) *fallbacks = h->ws.st_fallback;
+  std::lock_guard<std::mutex> g(h->last_mu);
+  if (emitted) *emitted = h->last_emitted;
+  if (rescored) *rescored = h->last_rescored;
+  if (fallbacks) *fallbacks = h->last_fallback;
   return VSG_OK;
 }
 
@@ -1169,6 +1348,13 @@ int vsg_synchronize(vsg_index* h) {
   if (!h) return fail(VSG_ERR_INVALID_ARGUMENT, "null index");
   VSG_HIP(hipSetDevice(h->device));
   VSG_HIP(hipStreamSynchronize(h->stream));
+  std::vector<hipEvent_t> evs;
+  {
+    std::lock_guard<std::mutex> g(h->lanes_mu);
+    for (auto& l : h->lanes)
+      if (l->pending) evs.push_back(l->done);
+  }
+  for (hipEvent_t ev : evs) VSG_HIP(hipEventSynchronize(ev));
   return VSG_OK;
 }
 
@@ -1578,14 +1764,13 @@ int vsg_graph_create(vsg_index* h, const float* d_queries, size_t nq, size_t k, 
   }
   e = hipStreamBeginCapture(gr->st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return cleanup(fail_hip("hipStreamBeginCapture", e));
-  const bool timing = h->timing;
-  h->timing = false;  // no event records inside the graph
+  t_capturing = true;  // no event records inside the graph
   if (p)
     rc = enqueue_short(h, gr->w, d_queries, nq, k, p, d_ids, d_dists, gr->st, &fallback);
   else
     rc = enqueue_exact(h, gr->w, d_queries, nq, k, method == VSG_METHOD_AUTO ? VSG_METHOD_SKINNY : method,
                        nullptr, d_ids, d_dists, gr->st);
-  h->timing = timing;
+  t_capturing = false;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(gr->st, &graph);
   gr->graph = graph;
@@ -1735,35 +1920,22 @@ int vsg_add_hnsw_file(vsg_index* h, const char* path, size_t* rows_added) {
   if (size < kHnswHeaderBytes + hd.n * hd.dim * sizeof(float))
     return fail(VSG_ERR_INVALID_ARGUMENT, std::string("truncated vector block: ") + path);
   f.seekg(std::streamoff(kHnswHeaderBytes), std::ios::beg);
-  {
-    std::lock_guard<std::mutex> g(h->mu);
-    std::unique_lock<std::shared_mutex> wl(h->rw);
-    if (h->n + hd.n >= VSG_NONE) return fail(VSG_ERR_LENGTH, "an index holds at most 2^32 - 2 vectors");
-    VSG_HIP(hipSetDevice(h->device));
-    rc = grow(h, h->n + size_t(hd.n));  // one allocation for the whole block
-    if (rc) return rc;
-  }
-  // stream the block in slices of ~64 MB: read into pinned memory, then append
-  const size_t per_slice = std::max<size_t>(1, (size_t(64) << 20) / (h->dim * sizeof(float)));
-  HostBuf<float> stage;
-  hipError_t e = stage.reserve(std::min<size_t>(per_slice, std::max<uint64_t>(1, hd.n)) * h->dim);
-  if (e != hipSuccess) return fail_hip("pinned staging", e);
-  for (uint64_t r0 = 0; r0 < hd.n; r0 += per_slice) {
-    const size_t m = size_t(std::min<uint64_t>(per_slice, hd.n - r0));
-    f.read(reinterpret_cast<char*>(stage.p), std::streamsize(m * h->dim * sizeof(float)));
-    if (!f) {
-      stage.release();
-      return fail(VSG_ERR_INVALID_ARGUMENT, std::string("read failed: ") + path);
-    }
-    rc = vsg_add(h, stage.p, m);
-    if (rc) {
-      stage.release();
-      return rc;
-    }
-    if (rows_added) *rows_added += m;
-  }
-  stage.release();
-  return VSG_OK;
+  std::lock_guard<std::mutex> g(h->mu);
+  std::unique_lock<std::shared_mutex> wl(h->rw);
+  if (h->n + hd.n >= VSG_NONE) return fail(VSG_ERR_LENGTH, "an index holds at most 2^32 - 2 vectors");
+  if (hd.n == 0) return VSG_OK;
+  VSG_HIP(hipSetDevice(h->device));
+  rc = grow(h, h->n + size_t(hd.n));  // one allocation for the whole block
+  if (rc) return rc;
+  // the block is read slice by slice straight into the pinned upload slices; a failed read
+  // leaves the index as it was
+  rc = upload_rows(h, size_t(hd.n), [&](float* dst, size_t, size_t c) {
+    f.read(reinterpret_cast<char*>(dst), std::streamsize(c * h->dim * sizeof(float)));
+    if (!f) return fail(VSG_ERR_INVALID_ARGUMENT, std::string("read failed: ") + path);
+    return VSG_OK;
+  });
+  if (rc == VSG_OK && rows_added) *rows_added = size_t(hd.n);
+  return rc;
 }
 
 int vsg_search_exact_below(vsg_index* h, const float* queries, size_t nq, size_t k,

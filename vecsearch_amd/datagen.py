@@ -5,7 +5,11 @@
   centres depend only on ``centers_seed`` so base rows and held-out queries share clusters.
 - ``unit_vectors(n, d, seed)``: i.i.d. Gaussian rows, normalised (the reference benchmark's
   data, bench/gpu_bench.py:25-30).
-- ``zipf_groups(n, n_groups, s, seed)``: one group label per row with P(rank r) ~ r^-s.
+- ``zipf_groups(n, n_groups, s, seed, shift, cutoff)``: one group label per row with
+  P(rank r) ~ (r + shift)^-s * exp(-(r - 1) / cutoff) (plain Zipf with shift 0, no cutoff).
+- ``filtered_groups(n, seed)``: the filtered-scan workload's labels: 100 groups, Zipf exponent
+  1.1 with shift 5.56 and cutoff 24.54, so the expected group sizes run from 12 % of the rows
+  (rank 1) down to 0.01 % (rank 100). Plain Zipf(1.1) over 100 ranks spans 23.4 % .. 0.15 %.
 - ``group_csr(groups, n_groups)``: per-group sorted row ids as CSR (offsets, ids).
 - ``query_filters`` / ``query_ranges``: a random group and k per query, as ranges into the CSR.
 Rows are generated in slices so float64 temporaries stay small.
@@ -62,10 +66,27 @@ def unit_vectors(n, d, seed, slice_rows=100_000):
     return _normalize_(x)
 
 
-def zipf_groups(n, n_groups=100, s=1.1, seed=2):
-    p = 1.0 / np.arange(1, n_groups + 1) ** s
-    p /= p.sum()
+def zipf_weights(n_groups=100, s=1.1, shift=0.0, cutoff=None):
+    """Probability of each rank 1..n_groups: (r + shift)^-s, times exp(-(r - 1) / cutoff)."""
+    r = np.arange(1, n_groups + 1, dtype=np.float64)
+    w = (r + shift) ** -s
+    if cutoff:
+        w *= np.exp(-(r - 1) / cutoff)
+    return w / w.sum()
+
+
+def zipf_groups(n, n_groups=100, s=1.1, seed=2, shift=0.0, cutoff=None):
+    p = zipf_weights(n_groups, s, shift, cutoff)
     return np.random.default_rng([seed, 13]).choice(n_groups, size=n, p=p).astype(np.int32)
+
+
+# The filtered workload's group law: the Zipf(1.1) rank exponent, with the rank shift and the
+# exponential cutoff solved so that rank 1 holds 12 % of the rows and rank 100 holds 0.01 %.
+F_GROUPS, F_S, F_SHIFT, F_CUTOFF = 100, 1.1, 5.5613, 24.5404
+
+
+def filtered_groups(n, seed=2):
+    return zipf_groups(n, F_GROUPS, F_S, seed, F_SHIFT, F_CUTOFF)
 
 
 def group_csr(groups, n_groups):

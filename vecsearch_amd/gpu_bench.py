@@ -1,160 +1,127 @@
-"""QPS-vs-batch sweep of the engine, written in the reference's CSV format.
+"""QPS by batch size for each search method, as CSV rows in the reference's result format.
 
-    python -m vecsearch_amd.gpu_bench --backends hip,hip-naive,hip-skinny,hip-bf16,hip-fp8 \\
-        --out results/gpu/mi355x.csv
+    python -m vecsearch_amd.gpu_bench --out results/gpu/mi355x.csv [--methods auto,skinny,bf16,fp8]
 
-Rows have the columns of bench/gpu_bench.py:121-133 (backend, device, n, dim, metric, k, batch,
-ms_per_batch, qps, recall_vs_cpu) and the same data and timing rules, so they read next to the
-reference's results/gpu/*.csv:
-  - base and queries: seeded i.i.d. Gaussian rows, L2-normalised (gpu_bench.py:25-30);
-  - each batch: one warm-up call, then the median of at least 3 calls and `--min-time`
-    seconds; a call is the host API (host queries in, host results out, gpu_bench.py:75-85).
-Backends: "hip" (method auto), "hip-naive", "hip-skinny", "hip-tiled", "hip-bf16", "hip-fp8";
-"cpu" (the reference's BruteForceIndex) when the reference's `vecsearch` package is importable.
-recall_vs_cpu is measured against that CPU index when it is available, else against the
-engine's exact streaming scan ("hip-naive"), whose ids are the CPU index's.
+The file has the ten columns of the reference's results/gpu/*.csv (bench/gpu_bench.py:121):
+backend, device, n, dim, metric, k, batch, ms_per_batch, qps, recall_vs_cpu. A backend is
+"hip-<method>" ("hip" for method auto). Two rules are kept so the rows compare with the
+reference's: the data are i.i.d. Gaussian rows, L2-normalised (datagen.unit_vectors), and a
+cell's time is the median over repeated host calls (queries in from host memory, results out
+to it) after one untimed call, repeated until there are at least three and `--min-time`
+seconds of them (bench/gpu_bench.py:75-85).
+
+recall_vs_cpu compares each method's ids with the exact streaming scan ("naive"), whose ids
+and distances equal the reference CPU index's bit for bit; a CPU timing baseline is bench.py's.
 """
 import argparse
 import csv
 import os
-import platform
+import statistics
 import subprocess
 import time
+from dataclasses import asdict, dataclass
 
-import numpy as np
+from . import GpuBruteForceIndex, datagen, recall_at_k
 
-from . import GpuBruteForceIndex, normalize, recall_at_k
-
-BACKENDS = ["hip", "hip-naive", "hip-skinny", "hip-tiled", "hip-bf16", "hip-fp8", "cpu"]
-
-
-def unit_vectors(n, d, seed):
-    """gpu_bench.py:25-30: Gaussian rows drawn in slices of 100,000, then normalised."""
-    rng = np.random.default_rng(seed)
-    x = np.empty((n, d), np.float32)
-    for i in range(0, n, 100_000):
-        x[i:i + 100_000] = rng.standard_normal((min(100_000, n - i), d), dtype=np.float32)
-    return normalize(x)
+METHODS = ("auto", "naive", "skinny", "tiled", "bf16", "fp8")
 
 
-def cpu_name():
+@dataclass
+class Row:
+    backend: str
+    device: str
+    n: int
+    dim: int
+    metric: str
+    k: int
+    batch: int
+    ms_per_batch: float
+    qps: float
+    recall_vs_cpu: float
+
+
+COLUMNS = list(Row.__dataclass_fields__)
+
+
+def device_label():
+    """The GPU's marketing name from rocm-smi, else the architecture."""
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return platform.processor() or "cpu"
-
-
-def gpu_name():
-    try:
-        out = subprocess.run(["rocm-smi", "--showproductname"], capture_output=True, text=True,
-                             timeout=20).stdout
-        for line in out.splitlines():
-            if "Card Series" in line or "Card series" in line:
-                return line.split(":", 2)[-1].strip()
+        text = subprocess.run(["rocm-smi", "--showproductname"], capture_output=True, text=True,
+                              timeout=20).stdout
     except (OSError, subprocess.SubprocessError):
-        pass
-    return "AMD Instinct (gfx950)"
+        text = ""
+    names = [ln.split(":")[-1].strip() for ln in text.splitlines() if "card series" in ln.lower()]
+    return names[0] if names else "AMD Instinct (gfx950)"
 
 
-def reference_cpu_index():
-    """The reference's CPU BruteForceIndex, if its Python package is installed."""
-    try:
-        import vecsearch  # the reference package (python/vecsearch)
-    except ImportError:
-        return None
-    return getattr(vecsearch, "BruteForceIndex", None)
+def median_call_seconds(call, min_time):
+    """One untimed call, then timed calls until there are >= 3 and they sum to >= min_time
+    (a single call longer than min_time ends the cell after three)."""
+    call()
+    spent = []
+    while len(spent) < 3 or (sum(spent) < min_time and spent[0] <= min_time):
+        t0 = time.perf_counter()
+        call()
+        spent.append(time.perf_counter() - t0)
+    return statistics.median(spent)
 
 
-def make_backend(name, xb, metric, threads):
-    """Returns (search(q, k) -> ids, device description)."""
-    d = xb.shape[1]
-    if name == "cpu":
-        cls = reference_cpu_index()
-        if cls is None:
-            raise SystemExit("backend 'cpu' needs the reference's vecsearch package")
-        idx = cls(d, metric)
-        idx.add(xb)
-        return (lambda q, k: idx.search(q, k, num_threads=threads)[0]), f"{cpu_name()}, {threads} threads"
-    idx = GpuBruteForceIndex(d, metric)
-    idx.add(xb)
-    method = name.split("-", 1)[1] if "-" in name else "auto"
-    return (lambda q, k: idx.search(q, k, method=method)[0]), gpu_name()
-
-
-def time_batch(search, q, k, min_time):
-    """Median seconds per call: at least 3 calls and min_time seconds, unless one call is longer."""
-    search(q, k)
-    times = []
-    while len(times) < 3 or sum(times) < min_time:
-        t = time.perf_counter()
-        search(q, k)
-        times.append(time.perf_counter() - t)
-        if times[0] > min_time:
-            break
-    return float(np.median(times))
+def sweep(index, queries, k, methods, batches, caps, truth_rows, min_time, device):
+    """Yields one Row per (method, batch) cell; truth_rows are the exact ids of the first
+    len(truth_rows) queries."""
+    n, dim = len(index), index.dim
+    check = queries[:len(truth_rows)]
+    for method in methods:
+        found = index.search(check, k, method)[0]
+        recall = recall_at_k(found, truth_rows, k)
+        for b in batches:
+            if b > caps.get(method, b):
+                continue
+            block = queries[:b]
+            sec = median_call_seconds(lambda: index.search(block, k, method), min_time)
+            yield Row("hip" if method == "auto" else f"hip-{method}", device, n, dim, index.metric, k, b,
+                      round(sec * 1e3, 3), round(b / sec, 1), round(recall, 4))
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--backends", default="hip,hip-skinny,hip-bf16,hip-fp8", help=f"comma list of {BACKENDS}")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--methods", default="auto,skinny,bf16,fp8", help=f"comma list of {', '.join(METHODS)}")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=384)
-    ap.add_argument("--metric", default="ip", choices=["ip", "l2"])
+    ap.add_argument("--metric", choices=("ip", "l2"), default="ip")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--batches", default="1,8,32,128,512,1024,4096")
-    ap.add_argument("--max-batch", default="cpu=1024,hip-naive=1024",
-                    help="per-backend batch caps (backend=batch, comma separated)")
-    ap.add_argument("--threads", type=int, default=os.cpu_count())
+    ap.add_argument("--caps", default="naive=1024", help="method=largest batch, comma separated")
     ap.add_argument("--min-time", type=float, default=1.0)
-    ap.add_argument("--check", type=int, default=128, help="queries compared with the truth")
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--check", type=int, default=128, help="queries whose recall is measured")
     args = ap.parse_args(argv)
 
-    backends = args.backends.split(",")
-    unknown = set(backends) - set(BACKENDS)
-    if unknown:
-        ap.error(f"unknown backends {sorted(unknown)}")
-    batches = [int(b) for b in args.batches.split(",")]
-    caps = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.max_batch.split(",") if kv}
+    methods = [m for m in args.methods.split(",") if m]
+    bad = [m for m in methods if m not in METHODS]
+    if bad:
+        ap.error(f"unknown methods {bad}")
+    batches = sorted(int(b) for b in args.batches.split(","))
+    caps = dict((m, int(b)) for m, b in (kv.split("=") for kv in args.caps.split(",") if kv))
 
-    print(f"data: {args.n:,} x {args.dim} ({args.metric}), queries: {max(batches):,}", flush=True)
-    xb = unit_vectors(args.n, args.dim, 0)
-    xq = unit_vectors(max(max(batches), args.check), args.dim, 1)
-
-    cls = reference_cpu_index()
-    if cls is not None:
-        cpu = cls(args.dim, args.metric)
-        cpu.add(xb)
-        truth = cpu.search(xq[:args.check], args.k, num_threads=args.threads)[0]
-        del cpu
-    else:  # the engine's exact scan returns the CPU index's ids
-        exact = GpuBruteForceIndex(args.dim, args.metric)
-        exact.add(xb)
-        truth = exact.search(xq[:args.check], args.k, method="naive")[0]
-        del exact
+    base = datagen.unit_vectors(args.n, args.dim, seed=0)
+    queries = datagen.unit_vectors(max(batches[-1], args.check), args.dim, seed=1)
+    index = GpuBruteForceIndex(args.dim, args.metric)
+    index.add(base)
+    del base
+    truth = index.search(queries[:args.check], args.k, "naive")[0]
+    device = device_label()
+    print(f"{args.n:,} x {args.dim} {args.metric}, k={args.k}, on {device}", flush=True)
 
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-    fields = ["backend", "device", "n", "dim", "metric", "k", "batch", "ms_per_batch", "qps", "recall_vs_cpu"]
     with open(args.out, "w", newline="") as f:
-        w = csv.DictWriter(f, fieldnames=fields)
-        w.writeheader()
-        for name in backends:
-            search, device = make_backend(name, xb, args.metric, args.threads)
-            recall = recall_at_k(search(xq[:args.check], args.k), truth, args.k)
-            print(f"{name} on {device}: recall vs cpu {recall:.4f}", flush=True)
-            for b in batches:
-                if b > caps.get(name, b):
-                    continue
-                sec = time_batch(search, xq[:b], args.k, args.min_time)
-                w.writerow(dict(backend=name, device=device, n=args.n, dim=args.dim, metric=args.metric,
-                                k=args.k, batch=b, ms_per_batch=round(sec * 1e3, 3), qps=round(b / sec, 1),
-                                recall_vs_cpu=round(recall, 4)))
-                f.flush()
-                print(f"  batch {b:>5}: {sec * 1e3:10.2f} ms  {b / sec:12,.0f} QPS", flush=True)
-            del search
+        out = csv.DictWriter(f, fieldnames=COLUMNS)
+        out.writeheader()
+        for row in sweep(index, queries, args.k, methods, batches, caps, truth, args.min_time, device):
+            out.writerow(asdict(row))
+            f.flush()
+            print(f"{row.backend:>10} batch {row.batch:>5}: {row.ms_per_batch:9.3f} ms {row.qps:12,.0f} QPS "
+                  f"recall {row.recall_vs_cpu}", flush=True)
     print(f"wrote {args.out}")
 
 
