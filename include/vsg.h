@@ -95,6 +95,23 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
 int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
                       const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The search of
+ * vsg_search_device (same arguments, same ids and distances for every method), answered by the
+ * exact streaming scan whose only scratch memory is a static device array of
+ * VSG_NOALLOC_SCRATCH_BYTES per GPU, loaded with the library and held until the process ends;
+ * the call makes no device allocation and creates no stream or event. The scratch holds
+ * q x W x k 8-byte keys for the q <= 8 queries of one pass (W = the scan's lists, at most 512
+ * at the default 8 waves per CU on 256 CUs): a search whose single pass needs more (e.g.
+ * scan_waves_per_cu 32 with k = 128) is rejected with VSG_ERR_INVALID_ARGUMENT. Larger batches
+ * run in rounds. Every call on one GPU shares its array, and each call leaves it in its own
+ * state: calls that overlap on one GPU (other streams, other threads, other indexes) race;
+ * the caller orders them. The index must have been created (vsg_create sets the array up on
+ * its device). */
+#define VSG_NOALLOC_SCRATCH_BYTES (4u << 20)
+int vsg_search_device_noalloc(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
+                              const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream);
+
 /* Batched filtered exact scan with a per-query k and per-query allowed-id lists: query i may
  * return only rows allow_ids[range[2i] .. range[2i+1]) (strictly increasing, < size()), and
  * returns ks[i] <= kmax results. Ranges may coincide: queries with identical ranges share

@@ -32,6 +32,12 @@
 #include "vsg.h"
 #include "vsg_internal.h"
 
+// The list space of vsg_search_device_noalloc: a fixed device array, one copy per GPU, held
+// until the process ends. Host code never passes `scratch` itself to a kernel (on the host the
+// name is only a stand-in for the symbol); it passes the device address that
+// hipGetSymbolAddress returned for the current device (scratch_address).
+static __device__ __align__(256) uint64_t scratch[VSG_NOALLOC_SCRATCH_BYTES / sizeof(uint64_t)];
+
 namespace {
 
 thread_local std::string g_err;
@@ -51,6 +57,27 @@ int fail_hip(const char* what, hipError_t e) {
     hipError_t e_ = (expr);                            \
     if (e_ != hipSuccess) return fail_hip(#expr, e_);  \
   } while (0)
+
+// Device address of `scratch` on each device, looked up once per device (with that device
+// current) by the first vsg_create on it.
+std::mutex g_scratch_mu;
+std::map<int, uint64_t*> g_scratch_addr;
+
+int scratch_setup(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_addr.count(device)) return VSG_OK;
+  void* p = nullptr;
+  VSG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(VSG_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+  g_scratch_addr[device] = static_cast<uint64_t*>(p);
+  return VSG_OK;
+}
+
+uint64_t* scratch_address(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  return it == g_scratch_addr.end() ? nullptr : it->second;
+}
 
 template <class T>
 struct DevBuf {
@@ -511,16 +538,21 @@ struct IndexTiming : vsg::TimingSink {
   }
 };
 
-// Enqueue an exact streaming search of nq device-resident queries on stream st.
-int enqueue_exact(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
-                  const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
+// Enqueue an exact streaming search of nq device-resident queries on stream st, its per-block
+// lists in `part` (part_keys keys: every pass writes all W x k keys of its queries, so nothing
+// in it needs clearing). Queries go in rounds of as many passes as `part` holds.
+int enqueue_exact_in(vsg_index* h, uint64_t* part, size_t part_keys, const float* d_q, size_t nq, size_t k,
+                     int method, const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
   const ScanPlan p = plan_scan(h);
   const int qb = queries_per_pass(method, nq, h->dp);
-  const size_t per_query = size_t(p.W) * k * sizeof(uint64_t);
-  size_t round = std::max<size_t>(qb, kPartBudgetBytes / per_query / qb * qb);
+  const size_t per_query = size_t(p.W) * k;
+  size_t round = part_keys / per_query / qb * qb;
+  if (round == 0)
+    return fail(VSG_ERR_INVALID_ARGUMENT, "exact scan: one pass needs " + std::to_string(qb * per_query * 8) +
+                                              " bytes of list space, more than the " +
+                                              std::to_string(part_keys * 8) + " available");
   round = std::min(round, (nq + qb - 1) / qb * qb);
-  hipError_t e = w.part.reserve(round * size_t(p.W) * k);
-  if (e != hipSuccess) return fail_hip("workspace", e);
+  hipError_t e;
   const bool l2 = h->metric == VSG_METRIC_L2;
   for (size_t r0 = 0; r0 < nq; r0 += round) {
     const size_t rq = std::min(round, nq - r0);
@@ -531,20 +563,34 @@ int enqueue_exact(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
         const uint32_t nqp = uint32_t(std::min<size_t>(qb, rq - q0));
         e = vsg::launch_scan(qb, l2, uint32_t(k), p.n_blocks, st, h->tiles, uint32_t(h->n),
                              uint32_t(h->dp), uint32_t(h->dim), qr, uint32_t(q0), nqp,
-                             p.rows_per_wave, d_mask, w.part.p);
+                             p.rows_per_wave, d_mask, part);
         if (e != hipSuccess) return fail_hip("scan kernel", e);
         ts.launches++;
       }
     }
     {
       TimedScope ts(h, st, VSG_KERNEL_MERGE);
-      e = vsg::launch_merge(uint32_t(rq), w.part.p, nullptr, p.W, uint32_t(k), nullptr,
+      e = vsg::launch_merge(uint32_t(rq), part, nullptr, p.W, uint32_t(k), nullptr,
                             uint32_t(k), nullptr, d_ids + r0 * k, d_dists + r0 * k, uint32_t(k), st);
       if (e != hipSuccess) return fail_hip("merge kernel", e);
       ts.launches++;
     }
   }
   return VSG_OK;
+}
+
+// The same with the lists in the workspace (up to kPartBudgetBytes per round).
+int enqueue_exact(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, int method,
+                  const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, hipStream_t st) {
+  const ScanPlan p = plan_scan(h);
+  const int qb = queries_per_pass(method, nq, h->dp);
+  const size_t per_query = size_t(p.W) * k * sizeof(uint64_t);
+  size_t round = std::max<size_t>(qb, kPartBudgetBytes / per_query / qb * qb);
+  round = std::min(round, (nq + qb - 1) / qb * qb);
+  const size_t keys = round * size_t(p.W) * k;
+  hipError_t e = w.part.reserve(keys);
+  if (e != hipSuccess) return fail_hip("workspace", e);
+  return enqueue_exact_in(h, w.part.p, keys, d_q, nq, k, method, d_mask, d_ids, d_dists, st);
 }
 
 // ---- certified shortlist (vsg_short.hip) ---------------------------------------------------
@@ -1029,6 +1075,8 @@ int vsg_create(size_t dim, int metric, int device, vsg_index** out) {
   if (device < 0 || device >= count)
     return fail(VSG_ERR_INVALID_ARGUMENT, "device index out of range");
   VSG_HIP(hipSetDevice(device));
+  int rc = scratch_setup(device);
+  if (rc) return rc;
   vsg_index* h = new (std::nothrow) vsg_index;
   if (!h) return fail(VSG_ERR_OUT_OF_MEMORY, "host allocation failed");
   h->dim = dim;
@@ -1185,6 +1233,31 @@ int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k,
   LaneLease lease(h);
   if ((rc = lease.acquire(st))) return rc;
   return enqueue_search(h, lease.w(), d_queries, nq, k, method, d_mask, d_ids, d_dists, st);
+}
+
+int vsg_search_device_noalloc(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
+                              const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream) {
+  int rc = check_search_args(h, k, method);
+  if (rc) return rc;
+  if (nq == 0 || k == 0) return VSG_OK;
+  if (!d_queries || !d_ids || !d_dists) return fail(VSG_ERR_INVALID_ARGUMENT, "null buffer");
+  std::shared_lock<std::shared_mutex> rl(h->rw);
+  VSG_HIP(hipSetDevice(h->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  if (h->n == 0) {
+    VSG_HIP(vsg::launch_fill_missing(st, d_ids, d_dists, nq * k));
+    return VSG_OK;
+  }
+  uint64_t* part = scratch_address(h->device);
+  if (!part) return fail(VSG_ERR_DEVICE, "no scratch address for this device");
+  // every method has the exact streaming scan's results; NAIVE keeps one query per pass
+  const int m = method == VSG_METHOD_NAIVE ? VSG_METHOD_NAIVE : VSG_METHOD_SKINNY;
+  const bool timing = t_capturing;
+  t_capturing = true;  // no timing events (creating one may allocate)
+  rc = enqueue_exact_in(h, part, VSG_NOALLOC_SCRATCH_BYTES / sizeof(uint64_t), d_queries, nq, k, m, d_mask,
+                        d_ids, d_dists, st);
+  t_capturing = timing;
+  return rc;
 }
 
 int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32_t* ks,
