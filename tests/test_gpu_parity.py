@@ -202,6 +202,33 @@ def test_search_batch_eight_lanes_per_row(d, qb, r, metric):
     assert_same(got, oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=100))
 
 
+@pytest.mark.parametrize("d", [64, 128, 384, 768])
+@pytest.mark.parametrize("kset", [(1, 10, 100), (5, 64)])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_search_batch_one_lane_per_row(d, kset, metric):
+    # one lane per row, queries as scalar operands, eight waves per item, lists merged by rank;
+    # 70 queries over 8 groups give items of 8 and of fewer queries; the 2,048-row chunks send
+    # whole 64-key steps through the sort-and-merge insert; 300 duplicated rows tie exactly
+    n = 14000
+    x = dg.mixture(n, d, seed=38)
+    x[7000:7300] = x[100]
+    groups = dg.zipf_groups(n, 8, 1.1, seed=39)
+    off, ids_g = dg.group_csr(groups, 8)
+    q = dg.mixture(70, d, seed=40)
+    q[3] = x[100]
+    ks, qg = dg.query_filters(70, 8, seed=41, ks=kset)
+    ranges = dg.query_ranges(qg, off)
+    ranges[::7] = [vs.ALL_ROWS, 0]
+    gpu = vs.GpuBruteForceIndex(d, metric)
+    gpu.add(x)
+    for chunk in (2048, 320):
+        gpu.set_param("lists_lanes_per_row", 1)
+        gpu.set_param("lists_chunk", chunk)
+        got = gpu.search_batch(q, ks, ranges, ids_g)
+        assert gpu.get_param("lists_last_lanes_per_row") == 1
+        assert_same(got, oracle.search_lists(x, q, ks, ranges, ids_g, metric, threads=8, kmax=int(max(kset))))
+
+
 def test_search_batch_autotuner_keeps_results():
     # the first large call of a shape class times every candidate tile on its own inputs; the
     # class then reuses the fastest; results are the same bits whichever tile runs

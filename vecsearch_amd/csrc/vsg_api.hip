@@ -1006,11 +1006,13 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
                                     g_err, &tm, tuned, &w.lists_last);
     if (allow_tune && pairs >= (uint64_t(1) << 24)) {
       // {queries per wave, row groups per lane step, chunk rows (0: planner), lanes per row}
-      // (32 queries: one item per block, every row read shared by its four waves)
+      // (32 queries: one item per block, every row read shared by its four waves; one lane per
+      // row: eight queries per wave through scalar loads)
       static const int cand[][4] = {{4, 4, 0, 4},     {4, 4, 1024, 4},  {4, 4, 4096, 4}, {8, 2, 1024, 4},
                                     {2, 4, 0, 4},     {8, 4, 1024, 8},  {8, 4, 4096, 8}, {8, 2, 2048, 8},
                                     {16, 2, 2048, 8}, {16, 2, 4096, 8}, {32, 8, 0, 8},   {32, 8, 2048, 8},
-                                    {32, 8, 8192, 8}};
+                                    {32, 8, 8192, 8}, {8, 1, 0, 1},     {8, 1, 1024, 1}, {8, 1, 2048, 1},
+                                    {8, 1, 4096, 1}};
       vsg::ListsTiles best;
       float best_ms = -1.f;
       for (const auto& c : cand) {
@@ -1343,8 +1345,8 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
     return VSG_OK;
   }
   if (key == "lists_lanes_per_row") {
-    if (value != 0 && value != 4 && value != 8)
-      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_lanes_per_row must be 0 (auto), 4 or 8");
+    if (value != 0 && value != 1 && value != 4 && value != 8)
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_lanes_per_row must be 0 (auto), 1, 4 or 8");
     h->lists_tiles.lanes_per_row = int(value);
     return VSG_OK;
   }

@@ -92,9 +92,9 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {
-  void* dev[10] = {};     // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out,
-                          // per-query thresholds, merge query map)
-  size_t dev_cap[10] = {};
+  void* dev[11] = {};     // device buffers (queries, ks, items, qperm, slot offsets, ids, part, out,
+                          // per-query thresholds, merge query map, queries in item order)
+  size_t dev_cap[11] = {};
   void* host[3] = {};     // pinned staging (queries, ids out, dists out)
   size_t host_cap[3] = {};
   uint32_t* counter = nullptr;

@@ -383,6 +383,84 @@ __device__ __forceinline__ void block_rank_merge(const uint64_t* lds, uint32_t k
   }
 }
 
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), m, 64));
+  const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), m, 64));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// value of lane `src` (0..63, per lane) of v: ds_bpermute, no LDS storage
+__device__ __forceinline__ uint64_t bperm64(uint64_t v, int src) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_ds_bpermute(src << 2, int(uint32_t(v))));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_ds_bpermute(src << 2, int(uint32_t(v >> 32))));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Sorts the wave's 64 keys (one per lane) ascending across the lanes: a bitonic network of 21
+// compare-exchange stages.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v, int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = shfl_xor64(v, j);
+      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = keep_min ? (o < v ? o : v) : (o < v ? v : o);
+    }
+  return v;
+}
+
+// Slot q (0 <= q < KP) of a wave list: register q >> 6, lane q & 63, read by every lane.
+template <int KP>
+__device__ __forceinline__ uint64_t list_at(const uint64_t (&e)[KP / 64], int q) {
+  uint64_t v = bperm64(e[0], q & 63);
+#pragma unroll
+  for (int r = 1; r < KP / 64; ++r) {
+    const uint64_t w = bperm64(e[r], q & 63);
+    v = (q >> 6) == r ? w : v;
+  }
+  return v;
+}
+
+// Merges 64 sorted keys (b: key of rank `lane`, kPadKey for none) into a sorted wave list and
+// keeps its KP smallest; returns the new k-th key. Merge path: output slot p takes the p-th
+// smallest of the union; i = the number of b keys among the outputs before p is the largest i
+// in [0, min(p, 64)] with i == 0 or b[i - 1] < list[p - i] (the predicate is monotone; list keys
+// come first among equals, and only pad keys can be equal). Every lane runs the same seven
+// search steps (a range of at most 65 values), because a lane permute reads zero from a lane
+// that is not active: nothing here may run under a partial mask.
+template <int KP>
+__device__ __forceinline__ uint64_t merge_sorted64(WaveList<KP>& lst, uint64_t b, int lane, int k) {
+  uint64_t out[KP / 64];
+#pragma unroll
+  for (int r = 0; r < KP / 64; ++r) {
+    const int p = r * 64 + lane;
+    int lo = 0, hi = p < 64 ? p : 64;
+#pragma unroll
+    for (int it = 0; it < 7; ++it) {
+      const int mid = (lo + hi + 1) >> 1;  // == lo once lo == hi
+      const uint64_t bv = bperm64(b, mid > 0 ? mid - 1 : 0);
+      const uint64_t av = list_at<KP>(lst.e, p - mid);  // 0 <= p - mid < KP
+      const bool pred = bv < av;
+      if (lo < hi) {
+        if (pred) lo = mid;
+        else hi = mid - 1;
+      }
+    }
+    const uint64_t bv = bperm64(b, lo < 64 ? lo : 63);
+    const uint64_t av = list_at<KP>(lst.e, p - lo);
+    out[r] = (lo < 64 && bv < av) ? bv : av;
+  }
+#pragma unroll
+  for (int r = 0; r < KP / 64; ++r) lst.e[r] = out[r];
+  const int kk = k - 1;
+  uint64_t thr = kPadKey;
+#pragma unroll
+  for (int r = 0; r < KP / 64; ++r)
+    if ((kk >> 6) == r) thr = readlane64(lst.e[r], kk & 63);
+  return thr;
+}
+
 // Offer keys of the calling wave to a query's k-list; `key` is kPadKey on lanes that carry no
 // candidate. Returns the updated admission threshold (the list's k-th key).
 template <int KP>
@@ -400,6 +478,24 @@ __device__ __forceinline__ uint64_t offer(WaveList<KP>& lst, uint64_t thr, uint6
     }
   }
   return thr;
+}
+
+}  // namespace vsg
+
+namespace vsg {
+
+// Offer the wave's 64 keys (one per lane, kPadKey on lanes without one) to a query's k-list:
+// a few admitted keys are inserted one at a time, more at once by a sort and a merge (about
+// the cost of 12 single inserts for 128 slots, 8 for 64). Either way the list ends up holding
+// its KP smallest keys among the old list and the keys below thr, so its first k slots are the
+// same. Returns the updated admission threshold.
+template <int KP>
+__device__ __forceinline__ uint64_t offer64(WaveList<KP>& lst, uint64_t thr, uint64_t key, int lane, int k) {
+  const uint64_t m = ballot(key < thr);
+  if (__popcll(m) <= (KP == 128 ? 12 : 8)) return offer<KP>(lst, thr, key, lane, k);
+  const uint64_t sorted = wave_sort64(key < thr ? key : kPadKey, lane);
+  const uint64_t t = merge_sorted64<KP>(lst, sorted, lane, k);
+  return t < thr ? t : thr;
 }
 
 }  // namespace vsg

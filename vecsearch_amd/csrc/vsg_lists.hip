@@ -129,6 +129,241 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
   }
 }
 
+// ---- one lane per row ------------------------------------------------------------------------
+// A workgroup of 8 waves takes one work item at a time (a chunk of a group's rows and up to 8 of
+// its queries) and runs it in rounds of 512 rows, wave w computing rows 64w .. 64w + 63 of the
+// round. Each lane owns one row and computes its distances to the item's queries itself: the
+// reference's 16 accumulators per pair (accumulator j sums dims j, j + 16, ...) and the hsum256
+// tree (distance.h:18-27, 54-69), all in its registers. The queries are the same for every lane
+// and every wave, so they come through scalar loads (their 12 KB stay in the scalar cache for
+// the item) and enter the FMAs as scalar operands: no LDS traffic for them and no cross-lane
+// reduction. They are read in item order from the copy lists_gather_queries_kernel makes.
+//
+// Rows reach the lanes through LDS: a wave loads each 16-dim step of its 64 rows with four
+// LDS-DMA loads, four lanes per row (16 cache lines per instruction instead of 64), two steps
+// ahead in a three-slot ring, and each lane then reads its own row's 64 bytes back. A row's four
+// 16-byte chunks are stored XOR-swizzled by (row >> 2) & 3, so the 16 lanes of a read touch 16
+// different bank groups.
+//
+// The round's keys (512 rows x 8 queries) go to LDS; then wave b alone offers query b's keys to
+// the query's k-list, which lives in its registers for the whole item (offer64: one sorted merge
+// for many admitted keys). One list per query sees every row of the chunk, so the inserts are
+// those of one sequential scan. At the end of the item wave b writes the list as the query's
+// chunk list and lowers the query's device-wide threshold, as the kernels above do; a workgroup
+// runs its items one after another, so its later chunks start from it.
+//
+// Items come from eight queues, one per XCD. Workgroups are dealt to the XCDs in turn, so
+// workgroup b starts on queue b % 8, where the planner put every query block of a chunk; a
+// chunk's re-reads then find its rows in that XCD's L2. A drained queue sends the workgroup on
+// to the next one.
+constexpr int kLaneQ = 8;        // queries per item
+constexpr int kLaneWaves = 8;    // waves per workgroup
+constexpr uint32_t kQueues = 8;  // one per XCD
+constexpr uint32_t kCounterWords = 32;  // [0] the single queue, [1..8] the XCD queues,
+constexpr uint32_t kBoundsAt = 16;      // [16..24] the XCD queues' item ranges
+constexpr uint32_t kLaneRound = 64 * kLaneWaves;                    // rows per round
+constexpr size_t kLaneKeyBytes = size_t(kLaneQ) * kLaneRound * 8;  // 32 KB
+constexpr size_t kLaneStage = 4096;                                // one step of 64 rows
+constexpr int kLaneSlots = 3;                                     // ring slots per wave
+constexpr size_t kLaneLds = kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage + 16;
+
+// Byte offset of 16-byte chunk j of row r (0..63) in a staged step.
+__device__ __forceinline__ uint32_t lane_slot(uint32_t r, uint32_t j) {
+  return (r >> 4) * 1024 + (r & 15) * 64 + ((j ^ ((r >> 2) & 3)) << 4);
+}
+
+// One 16-dim step of NQ queries against the lane's row. The query values come in pairs of
+// queries (32 scalar registers): pair p + 1 is requested right after the first FMA of pair p,
+// which is where the wait for pair p falls, so the remaining 31 FMAs of pair p cover the next
+// pair's load. (Scalar loads complete out of order: any wait is a wait for all of them.)
+template <int NQ, bool L2>
+__device__ __forceinline__ void lane_fma_step(const float4 (&xv)[4], const float* __restrict__ qs, uint32_t d,
+                                              float (&acc)[NQ][16]) {
+  static_assert(NQ % 2 == 0, "queries in pairs");
+  float qv[2][32];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    qv[0][i] = qs[i];
+    qv[0][16 + i] = qs[size_t(d) + i];
+  }
+#pragma unroll
+  for (int p = 0; p < NQ / 2; ++p) {
+    const float* q = qv[p & 1];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int b = 2 * p + h;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float x4[4] = {xv[c].x, xv[c].y, xv[c].z, xv[c].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int j = 4 * c + e;
+          acc[b][j] = fma_term<L2>(q[16 * h + j], x4[e], acc[b][j]);
+          if (h == 0 && j == 0 && p + 1 < NQ / 2) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              qv[(p + 1) & 1][i] = qs[size_t(2 * p + 2) * d + i];
+              qv[(p + 1) & 1][16 + i] = qs[size_t(2 * p + 3) * d + i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Keys of the wave's 64 rows x NQ queries into key_s[b * kLaneRound + 64 * wave + lane].
+// src[c]: this lane's global address for DMA load c of step 0 (row 16c + lane / 4 of the
+// wave's rows, the chunk that lands at this lane's slot); stage: the wave's three 4 KB slots.
+template <int NQ, int D16, bool L2>
+__device__ __forceinline__ void lane_rows(const float* const (&src)[4], uint8_t* stage, const float* __restrict__ qb,
+                                          uint32_t d, uint32_t nd16, bool ok, uint32_t row, uint64_t* key_s,
+                                          int wave, int lane) {
+  float acc[NQ][16];
+#pragma unroll
+  for (int b = 0; b < NQ; ++b)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[b][j] = 0.f;
+  const uint32_t last = nd16 - 1;
+  auto issue = [&](uint32_t slot, uint32_t s) __attribute__((always_inline)) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) load_lds16(src[c] + 16 * s, stage + slot * kLaneStage + c * 1024);
+  };
+  issue(0, 0);
+  issue(1, last >= 1 ? 1 : last);
+  uint32_t slot = 0;  // slot of step s; step s + 2 goes to the slot read at step s - 1
+  for (uint32_t s = 0; s < nd16; ++s) {
+    issue(slot == 0 ? 2 : slot - 1, s + 2 <= last ? s + 2 : last);  // past the end: unread slots
+    wait_vm<8>();                                                   // step s has landed
+    const uint8_t* cur = stage + slot * kLaneStage;
+    slot = slot == 2 ? 0 : slot + 1;
+    float4 xv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xv[j] = *reinterpret_cast<const float4*>(cur + lane_slot(uint32_t(lane), uint32_t(j)));
+    lane_fma_step<NQ, L2>(xv, qb + 16 * s, d, acc);
+  }
+  wait_vm<0>();  // the extra loads past the last step land before the slots are reused
+  // the tree of hsum256 on (acc0 + acc1): v_j = a_j + a_{j+8}, w_c = v_c + v_{c+4},
+  // r = (w0 + w1) + (w2 + w3)
+#pragma unroll
+  for (int b = 0; b < NQ; ++b) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = acc[b][j] + acc[b][j + 8];
+    const float w0 = v[0] + v[4], w1 = v[1] + v[5], w2 = v[2] + v[6], w3 = v[3] + v[7];
+    const float r = (w0 + w1) + (w2 + w3);
+    key_s[b * kLaneRound + 64 * wave + lane] = ok ? make_key(L2 ? r : 1.0f - r, row) : kPadKey;
+  }
+}
+
+template <int D16, int KP, bool L2>
+__global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
+    const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ qsorted,
+    const uint32_t* __restrict__ ks, const ListItem* __restrict__ items, const uint32_t* __restrict__ qperm,
+    const uint32_t* __restrict__ allow_ids, const uint64_t* __restrict__ slot_off, uint32_t L,
+    uint64_t* __restrict__ part, uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t* key_s = reinterpret_cast<uint64_t*>(lds_raw);                // kLaneQ x kLaneRound
+  uint8_t* stage = lds_raw + kLaneKeyBytes + size_t(wave) * kLaneSlots * kLaneStage;  // this wave's ring
+  uint32_t* item_s =
+      reinterpret_cast<uint32_t*>(lds_raw + kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage);
+  const uint32_t nd16 = D16 > 0 ? uint32_t(D16) : d >> 4;
+  const uint32_t* bounds = counter + kBoundsAt;
+
+  uint32_t qu = blockIdx.x % kQueues, tries = 0;
+  for (;;) {
+    __syncthreads();  // the previous item's LDS is no longer read
+    if (tid == 0) {
+      uint32_t it = ~0u;
+      while (tries < kQueues) {
+        const uint32_t i = atomicAdd(counter + 1 + qu, 1u);
+        if (bounds[qu] + i < bounds[qu + 1]) {
+          it = bounds[qu] + i;
+          break;
+        }
+        qu = (qu + 1) % kQueues;  // this queue is drained: the next one
+        ++tries;
+      }
+      *item_s = it;
+    }
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(*item_s);  // the same in every lane
+    if (it == ~0u) break;
+    const ListItem w = items[it];
+    const uint32_t nq = w.nq;
+    const float* qb = qsorted + size_t(w.qoff) * d;  // the item's queries, consecutive
+
+    // wave b (< nq) owns query b's k-list and threshold for the whole item
+    const bool owner = uint32_t(wave) < nq;
+    const uint32_t myq = owner ? qperm[w.qoff + wave] : 0u;
+    const int kq = owner ? int(min(ks[myq], uint32_t(KP))) : 0;
+    // k == 0 admits nothing; else the tightest bound the query's other chunks published
+    uint64_t thr = kq > 0 ? __hip_atomic_load(qthr + myq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    WaveList<KP> lst;
+    lst.init();
+
+    for (uint32_t r0 = w.lb; r0 < w.le; r0 += kLaneRound) {
+      const uint32_t my0 = r0 + 64u * uint32_t(wave);
+      const uint32_t idx = my0 + uint32_t(lane);
+      const bool ok = idx < w.le;
+      if (my0 < w.le) {
+        const uint32_t row = w.direct ? (ok ? idx : w.lb) : allow_ids[ok ? idx : w.lb];
+        const float* src[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t rr = 16u * c + (uint32_t(lane) >> 2);  // row of the wave that DMA load c fills here
+          const uint32_t jj = (uint32_t(lane) & 3) ^ ((uint32_t(lane) >> 4) & 3);  // its chunk at this slot
+          const uint32_t rid = uint32_t(__shfl(int(row), int(rr), 64));
+          src[c] = base + size_t(rid) * dp + 4 * jj;
+        }
+        // the item's queries, rounded up to a pair
+        if (nq <= 2) {
+          lane_rows<2, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
+        } else if (kLaneQ <= 4 || nq <= 4) {
+          lane_rows<4, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
+        } else if constexpr (kLaneQ > 4) {
+          if (nq <= 6) lane_rows<6, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
+          else lane_rows<8, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
+        }
+      }
+      __syncthreads();  // the round's keys are in LDS
+      if (owner && kq > 0) {
+        const uint32_t nw = min(uint32_t(kLaneWaves), (w.le - r0 + 63) / 64);  // waves with rows
+#pragma unroll 1
+        for (uint32_t c = 0; c < nw; ++c) {
+          const uint64_t key = key_s[uint32_t(wave) * kLaneRound + 64 * c + uint32_t(lane)];
+          if (ballot(key < thr) == 0) continue;
+          thr = offer64<KP>(lst, thr, key, lane, kq);
+        }
+      }
+      __syncthreads();  // the keys are read before the next round writes them
+    }
+
+    if (owner) {
+      if (lane == 0 && thr != kPadKey && kq > 0) atomicMin(reinterpret_cast<unsigned long long*>(qthr + myq), thr);
+      uint64_t* dst = part + (slot_off[myq] + w.chunk) * L;
+#pragma unroll
+      for (int r = 0; r < KP / 64; ++r) {
+        const uint32_t sl = r * 64 + uint32_t(lane);
+        if (sl < L) dst[sl] = lst.e[r];
+      }
+    }
+  }
+}
+
+// The batch's queries in item order (qperm), so an item's queries are consecutive rows.
+__global__ void lists_gather_queries_kernel(const float* __restrict__ q, const uint32_t* __restrict__ qperm,
+                                            uint32_t nq, uint32_t d, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x;
+  if (i >= nq) return;
+  const float* src = q + size_t(qperm[i]) * d;
+  float* dst = out + size_t(i) * d;
+  for (uint32_t c = threadIdx.x; c < d; c += blockDim.x) dst[c] = src[c];
+}
+
 // ---- block-shared rows: 4 waves x 8 queries read every row once ---------------------------
 // The kernel above gives each wave its own item, so a group's rows are read from memory once
 // per QB queries. Here a work item belongs to a whole block: (group, chunk of rows, up to 32 of
@@ -326,7 +561,7 @@ __global__ __launch_bounds__(64 * kBlkWaves, 2) void lists_block_kernel(
 
 namespace {
 
-enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut, kDThr, kDQmap };
+enum { kDQ = 0, kDKs, kDItems, kDQperm, kDSlot, kDIds, kDPart, kDOut, kDThr, kDQmap, kDQsorted };
 
 hipError_t dev_reserve(ListsWorkspace& ws, int i, size_t bytes) {
   if (bytes <= ws.dev_cap[i]) return hipSuccess;
@@ -357,6 +592,7 @@ struct ListsArgs {
   const float* base;
   uint32_t dp, d;
   const float* q;
+  const float* qsorted;  // the queries in item order (one lane per row)
   const uint32_t* ks;
   const ListItem* items;
   uint32_t n_items;
@@ -424,7 +660,32 @@ hipError_t launch_block_k(bool l2, int kp, const ListsArgs& a) {
   return l2 ? launch_block_t<D16, 128, true>(a) : launch_block_t<D16, 128, false>(a);
 }
 
+template <int D16, int KP, bool L2>
+hipError_t launch_lane_t(const ListsArgs& a) {
+  const void* f = reinterpret_cast<const void*>(&lists_lane_kernel<D16, KP, L2>);
+  const size_t lds = kLaneLds;
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((lists_lane_kernel<D16, KP, L2>), a.grid, dim3(64 * kLaneWaves), lds, a.st, a.base, a.dp,
+                     a.d, a.qsorted, a.ks, a.items, a.qperm, a.ids, a.slot_off, a.L, a.part, a.counter, a.qthr);
+  return hipGetLastError();
+}
+
+template <int D16>
+hipError_t launch_lane_k(bool l2, int kp, const ListsArgs& a) {
+  if (kp == 64) return l2 ? launch_lane_t<D16, 64, true>(a) : launch_lane_t<D16, 64, false>(a);
+  return l2 ? launch_lane_t<D16, 128, true>(a) : launch_lane_t<D16, 128, false>(a);
+}
+
 hipError_t launch_lists_tile(int qb, int r, int lpr, bool l2, int kp, const ListsArgs& a) {
+  if (lpr == 1) {  // one lane per row (d a multiple of 32)
+    switch (a.d) {
+      case 384: return launch_lane_k<24>(l2, kp, a);
+      case 768: return launch_lane_k<48>(l2, kp, a);
+      case 128: return launch_lane_k<8>(l2, kp, a);
+      default: return launch_lane_k<0>(l2, kp, a);
+    }
+  }
   if (qb == kBlkItemQ) {  // block-shared rows (r = kBlkR, eight lanes per row)
     switch (a.d) {
       case 384: return launch_block_k<24>(l2, kp, a);
@@ -457,6 +718,7 @@ int hip_status(hipError_t e) {
 }  // namespace
 
 bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
+  if (lpr == 1) return r == 1 && dim % 32 == 0;  // kLaneQ queries per item, whatever qb
   if (qb == kBlkItemQ) return r == kBlkR && lpr == 8 && (dim == 128 || dim == 384 || dim == 768);
   if (lpr == 8)
     return (dim == 128 || dim == 384 || dim == 768) && ((qb == 8 && (r == 2 || r == 4)) || (qb == 16 && r == 2));
@@ -464,7 +726,7 @@ bool lists_tile_supported(int qb, int r, int lpr, size_t dim) {
 }
 
 void ListsWorkspace::release() {
-  for (int i = 0; i < 10; ++i) {
+  for (int i = 0; i < 11; ++i) {
     if (dev[i]) (void)hipFree(dev[i]);
     dev[i] = nullptr;
     dev_cap[i] = 0;
@@ -494,6 +756,7 @@ struct ListsPlanHost {
   std::vector<ListItem> items;
   std::vector<uint32_t> qperm;
   std::vector<uint64_t> slot_off;
+  uint32_t bounds[kQueues + 1] = {0};  // item range of each XCD queue
 };
 
 // Validates and plans a filtered batch, reserves its workspace and uploads the plan, ks and
@@ -565,6 +828,12 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
     return VSG_ERR_INVALID_ARGUMENT;
   }
   int lpr = tiles.lanes_per_row == 8 || block ? 8 : 4;
+  const bool lane_tile = !block && tiles.lanes_per_row == 1 && dim % 32 == 0;
+  if (lane_tile) {
+    lpr = 1;
+    qb = kLaneQ;
+    r = 1;
+  }
   if (block) r = kBlkR;
   if (!r) r = lpr == 8 ? (qb >= 16 ? 2 : 4) : (qb >= 8 ? 2 : 4);
   if (!lists_tile_supported(qb, r, lpr, dim)) {  // fall back to the nearest quad tile
@@ -579,7 +848,9 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
     const auto sp = span_of(kv.first);
     visits += (sp.second - sp.first) * ((kv.second.size() + qb - 1) / qb);
   }
-  const uint64_t wave_slots = uint64_t(num_cus) * (block ? 2 : 16);  // item slots
+  // item slots: workgroups of one item (block-shared rows, one lane per row: several items per
+  // workgroup in turn) or waves of one item each
+  const uint64_t wave_slots = uint64_t(num_cus) * (block ? 2 : lane_tile ? 3 : 16);
   uint64_t C = visits / (wave_slots * 2) + 1;
   C = std::min<uint64_t>(8192, std::max<uint64_t>(256, (C + 63) / 64 * 64));
   if (tiles.chunk) C = std::max<uint64_t>(64, (tiles.chunk + 63) / 64 * 64);
@@ -595,7 +866,7 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
     used->r = r;
     used->lanes_per_row = lpr;
     used->chunk = C;
-    used->blocks_per_cu = tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
+    used->blocks_per_cu = lane_tile ? 1 : tiles.blocks_per_cu > 0 ? tiles.blocks_per_cu : block ? 2 : 4;
   }
   std::vector<ListItem>& items = ph.items;
   std::vector<uint32_t>& qperm = ph.qperm;
@@ -605,8 +876,12 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   qperm.reserve(nq);
   slot_off.assign(nq + 1, 0);
   std::vector<uint32_t> nchunks_of(nq, 0);
-  // Items run chunk-major within a group: the query blocks of one chunk are pulled one after
-  // another, so the chunk's rows are re-read from cache rather than from HBM.
+  // The query blocks of one chunk are consecutive items, so its rows are re-read from cache
+  // rather than from HBM. Chunks are dealt to eight queues (one per XCD, for the kernels that
+  // use them), each to the queue with the least work so far; the items are then laid out queue
+  // by queue, and a kernel with one queue runs them in that order.
+  std::vector<std::vector<ListItem>> queue(kQueues);
+  std::vector<uint64_t> qwork(kQueues, 0);
   for (const auto& kv : groups) {
     const auto sp = span_of(kv.first);
     const uint64_t lb = sp.first, le = sp.second;
@@ -616,17 +891,26 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
     for (uint32_t q : qs) nchunks_of[q] = nch;
     const uint32_t qoff0 = uint32_t(qperm.size());
     for (uint32_t q : qs) qperm.push_back(q);
-    for (uint32_t c = 0; c < nch; ++c)
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t qi = uint32_t(std::min_element(qwork.begin(), qwork.end()) - qwork.begin());
+      const uint64_t clb = lb + uint64_t(c) * C, cle = std::min<uint64_t>(le, clb + C);
+      qwork[qi] += (cle - clb) * qs.size();
       for (size_t b0 = 0; b0 < qs.size(); b0 += qb) {
         ListItem it;
-        it.lb = uint32_t(lb + uint64_t(c) * C);
-        it.le = uint32_t(std::min<uint64_t>(le, lb + uint64_t(c + 1) * C));
+        it.lb = uint32_t(clb);
+        it.le = uint32_t(cle);
         it.chunk = c;
         it.qoff = qoff0 + uint32_t(b0);
         it.nq = uint32_t(std::min<size_t>(qb, qs.size() - b0));
         it.direct = direct;
-        items.push_back(it);
+        queue[qi].push_back(it);
       }
+    }
+  }
+  uint32_t bounds[kQueues + 1] = {0};
+  for (uint32_t qi = 0; qi < kQueues; ++qi) {
+    items.insert(items.end(), queue[qi].begin(), queue[qi].end());
+    bounds[qi + 1] = uint32_t(items.size());
   }
   for (size_t i = 0; i < nq; ++i) slot_off[i + 1] = slot_off[i] + nchunks_of[i];
   const uint32_t L = uint32_t(kmax);
@@ -648,7 +932,11 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   LISTS_HIP(dev_reserve(ws, kDIds, std::max<uint64_t>(1, total) * sizeof(uint32_t)));
   LISTS_HIP(dev_reserve(ws, kDPart, std::max<uint64_t>(1, n_slots) * L * sizeof(uint64_t)));
   LISTS_HIP(dev_reserve(ws, kDThr, nq * sizeof(uint64_t)));
-  if (!ws.counter) LISTS_HIP(hipMalloc(&ws.counter, sizeof(uint32_t)));
+  // + kLaneQ rows: an item of fewer queries still reads (and ignores) up to kLaneQ rows
+  if (lane_tile) LISTS_HIP(dev_reserve(ws, kDQsorted, (nq + kLaneQ) * dim * sizeof(float)));
+  if (!ws.counter) LISTS_HIP(hipMalloc(&ws.counter, kCounterWords * sizeof(uint32_t)));
+  std::memcpy(ph.bounds, bounds, sizeof(bounds));
+  LISTS_HIP(hipMemcpyAsync(ws.counter + kBoundsAt, ph.bounds, sizeof(ph.bounds), hipMemcpyHostToDevice, st));
 
   if (queries) {
     std::memcpy(ws.host[0], queries, qbytes);
@@ -669,8 +957,9 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   out.nq = uint32_t(nq);
   out.kmax = uint32_t(kmax);
   out.n_items = uint32_t(items.size());
-  out.grid = uint32_t(std::min<uint64_t>(want_blocks, uint64_t(num_cus) * bpc));
-  out.lds = block ? 0 : size_t(kWavesPerBlock) * qb * dp * sizeof(float);
+  out.grid = uint32_t(std::min<uint64_t>(lane_tile ? items.size() : want_blocks,
+                                          uint64_t(num_cus) * (lane_tile ? 1 : bpc)));
+  out.lds = (block || lane_tile) ? 0 : size_t(kWavesPerBlock) * qb * dp * sizeof(float);
   out.dp = uint32_t(dp);
   out.d = uint32_t(dim);
   out.qb = qb;
@@ -682,11 +971,11 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   return VSG_OK;
 }
 
-// counter = 0 and every per-query threshold = the pad key (a kernel, so that a captured step
-// holds no memset node)
+// work counters = 0 (the single queue's and the XCD queues') and every per-query threshold =
+// the pad key (a kernel, so that a captured step holds no memset node)
 __global__ void lists_reset_kernel(uint32_t* counter, uint64_t* qthr, uint32_t nq) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *counter = 0u;
+  if (i <= kQueues) counter[i] = 0u;
   if (i < nq) qthr[i] = ~uint64_t(0);
 }
 
@@ -699,6 +988,7 @@ ListsArgs lists_args(ListsWorkspace& ws, const ListsLaunch& L, const float* base
   a.dp = L.dp;
   a.d = L.d;
   a.q = d_q;
+  a.qsorted = static_cast<const float*>(ws.dev[kDQsorted]);
   a.ks = static_cast<const uint32_t*>(ws.dev[kDKs]);
   a.items = static_cast<const ListItem*>(ws.dev[kDItems]);
   a.n_items = L.n_items;
@@ -780,6 +1070,11 @@ hipError_t lists_enqueue(ListsWorkspace& ws, const ListsLaunch& L, const float* 
   if (e != hipSuccess) return e;
   if (L.n_items) {
     const ListsArgs a = lists_args(ws, L, base, d_q, st);
+    if (L.lpr == 1) {
+      hipLaunchKernelGGL(lists_gather_queries_kernel, dim3(L.nq), dim3(128), 0, st, d_q, a.qperm, L.nq, L.d,
+                         static_cast<float*>(ws.dev[kDQsorted]));
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (timing) timing->begin(st);
     if ((e = launch_lists_tile(L.qb, L.r, L.lpr, L.l2, L.kp, a)) != hipSuccess) return e;
     if (timing) timing->end(st, VSG_KERNEL_LISTS, 1);
