@@ -45,8 +45,9 @@ typedef struct vsg_index vsg_index;
  *   BF16    the same as TILED;
  *   FP8     the certified shortlist with an FP8 E4M3 (per-row scale) MFMA scan;
  *   AUTO    SKINNY for small batches, TILED for large ones.
- * The shortlist applies to inner product / cosine with dim <= 768 on bases of at least
- * 65,536 rows; elsewhere every method runs the exact scan. */
+ * The shortlist applies to inner product / cosine and to squared L2 (its estimate carries each
+ * row's |x|^2 / 2) with dim <= 768 on bases of at least 65,536 rows; elsewhere every method runs
+ * the exact scan. */
 #define VSG_METHOD_AUTO 0
 #define VSG_METHOD_NAIVE 1
 #define VSG_METHOD_SKINNY 2

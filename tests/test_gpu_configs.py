@@ -95,6 +95,29 @@ def test_q_shortlist(p2, prec, batch, k):
         np.testing.assert_array_equal(bits(d), bits(ed))
 
 
+@pytest.fixture(scope="module")
+def p2_l2(p2):
+    idx = vs.GpuBruteForceIndex(384, "l2")
+    idx.add(p2["x"])
+    ti, td = oracle.search(p2["x"], p2["q"][SAMPLE], 100, "l2", threads=THREADS)
+    yield dict(idx=idx, truth=(ti, td))
+    del idx
+
+
+@pytest.mark.parametrize("k", [10, 100])
+@pytest.mark.parametrize("batch", [128, 4096])
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_l2_shortlist_full_size(p2, p2_l2, prec, batch, k):
+    """Squared L2 at 1,000,000 x 384 through the certified shortlist."""
+    q = p2["q"][:batch]
+    ids, d = p2_l2["idx"].search(q, k, prec)
+    rows = SAMPLE[SAMPLE < batch]
+    check_rows(ids, d, rows, p2_l2["truth"][0][:len(rows)], p2_l2["truth"][1][:len(rows)], f"L2 {prec} b={batch} k={k}")
+    ei, ed = p2_l2["idx"].search(q, k, "skinny")
+    np.testing.assert_array_equal(ids, ei)
+    np.testing.assert_array_equal(bits(d), bits(ed))
+
+
 def test_q_fp64_error_bound(p2):
     """The returned distances are the FP32 values of the reference order; against FP64 their error
     stays within the sequential-sum bound gamma_384 * sum |q_i x_i| (unit rows: <= gamma_384)."""

@@ -433,9 +433,18 @@ def test_shortlist_grows_with_adds(mixture_70k):
     assert_same(gpu.search(q[:64], 10, method="bf16"), oracle.search(x, q[:64], 10, "ip", threads=8))
 
 
-def test_l2_large_batch_uses_exact_path():
-    x = dg.mixture(66_000, 64, seed=46)
-    q = dg.mixture(64, 64, seed=47)
-    gpu = vs.GpuBruteForceIndex(64, "l2")
+@pytest.mark.parametrize("method", ["bf16", "fp8", "auto"])
+@pytest.mark.parametrize("d,nq,k", [(64, 64, 10), (384, 300, 100), (384, 700, 10), (128, 40, 1), (768, 130, 37)])
+def test_l2_shortlist(method, d, nq, k):
+    # the L2 shortlist: estimates carry each row's |x|^2 / 2; rows of different norms (scaled by
+    # 0.5 .. 2) so that term matters; ids and distances are the exact scan's, and the shortlist
+    # (not only its exact fallback) answers some queries
+    x = dg.mixture(66_000, d, seed=46)
+    x *= np.exp(np.random.default_rng(48).uniform(np.log(0.5), np.log(2.0), len(x)))[:, None].astype(np.float32)
+    q = dg.mixture(nq, d, seed=47)
+    gpu = vs.GpuBruteForceIndex(d, "l2")
     gpu.add(x)
-    assert_same(gpu.search(q, 10, method="bf16"), oracle.search(x, q, 10, "l2", threads=8))
+    got = gpu.search(q, k, method)
+    assert_same(got, oracle.search(x, q, k, "l2", threads=8))
+    if method != "fp8":  # FP8's wider bound may send every query of a case to the exact scan
+        assert gpu.short_stats()["fallbacks"] < nq

@@ -235,6 +235,7 @@ struct vsg_index {
     void* frag = nullptr;             // fragment layout, cap rows
     float* scale = nullptr;           // fp8 per-row scale
     double* norms = nullptr;          // per row |x|^2, |x^|^2, |x - x^|^2
+    float* hn = nullptr;              // L2: per row |x|^2 / 2 (fp32)
     unsigned long long* maxes = nullptr;  // max |x|, |x^|, |x - x^| (fp64 bits)
     size_t rows = 0, cap = 0;
   } prec[3];
@@ -305,6 +306,7 @@ struct vsg_index {
       if (p.frag) (void)hipFree(p.frag);
       if (p.scale) (void)hipFree(p.scale);
       if (p.norms) (void)hipFree(p.norms);
+      if (p.hn) (void)hipFree(p.hn);
       if (p.maxes) (void)hipFree(p.maxes);
     }
     if (stream) (void)hipStreamDestroy(stream);
@@ -596,8 +598,8 @@ int enqueue_exact(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
 // ---- certified shortlist (vsg_short.hip) ---------------------------------------------------
 
 bool short_supported(const vsg_index* h, size_t k) {
-  return h->metric == VSG_METRIC_IP && vsg::short_ks(uint32_t(h->dim)) > 0 && h->n >= 65536 &&
-         k <= VSG_MAX_K && size_t(vsg::short_cand_cap()) >= 4 * k;
+  return vsg::short_ks(uint32_t(h->dim)) > 0 && h->n >= 65536 && k <= VSG_MAX_K &&
+         size_t(vsg::short_cand_cap()) >= 4 * k;
 }
 
 // Quantized copy `p` (1 bf16, 2 fp8) covering every row of the base. Built on the index
@@ -611,6 +613,7 @@ int ensure_prec(vsg_index* h, int p) {
     (void)hipFree(c.frag);
     if (c.scale) (void)hipFree(c.scale);
     (void)hipFree(c.norms);
+    if (c.hn) (void)hipFree(c.hn);
     (void)hipFree(c.maxes);
     c = vsg_index::Prec();
   }
@@ -622,6 +625,7 @@ int ensure_prec(vsg_index* h, int p) {
     VSG_HIP(hipMemsetAsync(c.frag, 0, bytes, h->stream));
     if (p == 2) VSG_HIP(hipMalloc(&c.scale, h->cap * sizeof(float)));
     VSG_HIP(hipMalloc(&c.norms, h->cap * 3 * sizeof(double)));
+    if (h->metric == VSG_METRIC_L2) VSG_HIP(hipMalloc(&c.hn, h->cap * sizeof(float)));
     VSG_HIP(hipMalloc(&c.maxes, 3 * sizeof(unsigned long long)));
     VSG_HIP(hipMemsetAsync(c.maxes, 0, 3 * sizeof(unsigned long long), h->stream));
     c.cap = h->cap;
@@ -631,6 +635,7 @@ int ensure_prec(vsg_index* h, int p) {
     VSG_HIP(vsg::launch_quantize(h->stream, h->base, uint32_t(h->dp), uint32_t(h->dim), uint32_t(c.rows),
                                  uint32_t(h->n), uint32_t(ks), p, c.frag, c.scale, c.norms));
     VSG_HIP(vsg::launch_norm_max(h->stream, c.norms, uint32_t(c.rows), uint32_t(h->n), c.maxes));
+    if (c.hn) VSG_HIP(vsg::launch_half_norms(h->stream, c.norms, uint32_t(c.rows), uint32_t(h->n), c.hn));
     c.rows = h->n;
   }
   VSG_HIP(hipStreamSynchronize(h->stream));
@@ -672,9 +677,11 @@ int short_reserve(vsg_index* h, Workspace& w, size_t nq, size_t k, int p, ShortP
   if ((e = w.s_nover.reserve(2)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
   // Pass 2 runs queries-stationary when the batch fits one block's registers (one round),
-  // else rows-stationary over 512-row tiles.
+  // else rows-stationary over 512-row tiles; L2 always rows-stationary (the pass that carries
+  // the rows' |x|^2 / 2 terms).
   uint32_t qs_blocks = 0, qs_groups = 0, qs_rows = 0, qs_qpr = 0;
-  const bool qs = vsg::short_qs_plan(p, ks, uint32_t(h->n), uint32_t(nq), h->num_cus, &qs_blocks, &qs_groups,
+  const bool qs = h->metric != VSG_METRIC_L2 &&
+                  vsg::short_qs_plan(p, ks, uint32_t(h->n), uint32_t(nq), h->num_cus, &qs_blocks, &qs_groups,
                                      &qs_rows, &qs_qpr);
   // pass-2 emission regions, one per wave: a region of R rows and Q queries receives about
   // Q * (candidates per query) * R / n entries (rows are in no particular order); sized for
@@ -734,13 +741,14 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
     float* dists = d_dists + r0 * k;
     // flags: [0] queries whose candidates overflowed, [1] emission regions that overflowed
     VSG_HIP(vsg::launch_query_prep(st, qr, uint32_t(h->dim), rq, uint32_t(ks), p, w.s_qfrag.p, w.s_qscale.p,
-                                   c.maxes, w.s_E.p, w.s_qaux.p, w.s_nover.p));
+                                   c.maxes, w.s_E.p, w.s_qaux.p, w.s_nover.p, h->metric == VSG_METRIC_L2));
 
     vsg::ShortScanArgs a;
     a.st = st;
     a.KS = ks;
     a.xfrag = c.frag;
     a.xscale = c.scale;
+    a.xhn = c.hn;
     a.n = uint32_t(h->n);
     a.qfrag = w.s_qfrag.p;
     a.qaux = w.s_qaux.p;

@@ -35,6 +35,7 @@ struct ShortScanArgs {
   int KS;
   const void* xfrag;
   const float* xscale;
+  const float* xhn = nullptr;  // L2: |x|^2 / 2 per row (the rows-stationary pass subtracts it)
   uint32_t n;
   const void* qfrag;
   const float* qaux;  // per query fragment: 16 thresholds, then 16 scales (query_aux_kernel)
@@ -75,7 +76,9 @@ hipError_t launch_norm_max(hipStream_t st, const double* norms, uint32_t r0, uin
 // queries [0, nq): fragments (out), scales, error bounds E, the pass-1 aux block; flags[0..1] = 0
 hipError_t launch_query_prep(hipStream_t st, const float* queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
                              void* out, float* qscale, const unsigned long long* maxes, float* E, float* aux,
-                             uint32_t* flags);
+                             uint32_t* flags, bool l2);
+// hn[r] = |x_r|^2 / 2 (fp32) from the copy's FP64 norms, rows [r0, r1)
+hipError_t launch_half_norms(hipStream_t st, const double* norms, uint32_t r0, uint32_t r1, float* hn);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n);
 hipError_t launch_mfma_scan(int prec, int pass, const ShortScanArgs& a);
 hipError_t launch_scatter(int prec, const ShortScanArgs& a);

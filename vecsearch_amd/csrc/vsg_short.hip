@@ -30,6 +30,18 @@
 // with gamma_n = n u / (1 - n u), u = 2^-24, m = ceil(d/16) + 12, and the row terms taken as
 // maxima over the base (|x|max, |x^|max, |x - x^|max). E is padded by 1 ulp-scale cushion.
 //
+// Squared L2 runs through the same passes. With hn(x) = |x|^2 / 2 (per row, rounded to fp32),
+// the estimate becomes dh(x) = 1 - (q^.x^ - hn(x)), and the reference distance d(x) maps to
+// t(x) = (d(x) - |q|^2 + 2) / 2 = 1 - q.x + |x|^2 / 2 + (d(x) - |q - x|^2) / 2, an increasing
+// function of d(x) for the query. The argument above then holds for t with the same dh:
+// ranking by t is ranking by (d, id), and |dh - t| <= E_L2, where E_L2 is the inner-product
+// bound's dot-product terms plus
+//   hn rounded to fp32                                              <= u |x|max^2 / 2
+//   the reference's l2_sq order vs the exact sum (halved)           <= gamma_m (|q| + |x|max)^2 / 2
+//   the subtraction of hn and the final 1 - s roundings             <= 4 u (1 + |q| |x| + |x|max^2)
+// The rows-stationary pass subtracts hn from every score (0 for inner product: the same bits);
+// L2 batches use that pass for pass 2 too.
+//
 // Data layouts (built from the row-major base when rows are added):
 //   BF16 copy: for row block rb (16 rows) and k-step s (32 dims) a 1 KB chunk; lane l of the
 //     chunk holds rows rb*16 + l%16, dims 32s + 8*(l/16) .. +7 -- exactly the B operand of
@@ -142,7 +154,7 @@ __global__ void norm_max_kernel(const double* __restrict__ norms, uint32_t r0, u
 
 // Per-query error bound E (see the header) from the query norms and the base maxima.
 __device__ __forceinline__ float query_error_bound(double qx2, double qh2, double qe2, int prec, uint32_t d,
-                                                   const unsigned long long* __restrict__ maxes) {
+                                                   const unsigned long long* __restrict__ maxes, bool l2) {
   const double X = __longlong_as_double((long long)maxes[0]);
   const double Xh = __longlong_as_double((long long)maxes[1]);
   const double dX = __longlong_as_double((long long)maxes[2]);
@@ -155,8 +167,14 @@ __device__ __forceinline__ float query_error_bound(double qx2, double qh2, doubl
   double e = nq_ * dX + dq * Xh;
   e += 2.0 * gd * nqh * Xh;
   if (prec == 2) e += (4.0 * u + 0x1p-10) * nqh * Xh;  // scales; the FP8 instruction's own sum
-  e += gm * nq_ * X;
-  e += 4.0 * u * (1.0 + nq_ * X);
+  if (l2) {
+    e += 0.5 * u * X * X;                       // hn in fp32
+    e += 0.5 * gm * (nq_ + X) * (nq_ + X);      // the reference's l2_sq order
+    e += 4.0 * u * (1.0 + nq_ * X + X * X);     // s - hn and 1 - s roundings
+  } else {
+    e += gm * nq_ * X;
+    e += 4.0 * u * (1.0 + nq_ * X);
+  }
   return float(e * (1.0 + 1e-6) + 1e-12);
 }
 
@@ -166,7 +184,7 @@ __global__ __launch_bounds__(256) void query_prep_kernel(
     const float* __restrict__ queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
     __bf16* __restrict__ out16, uint8_t* __restrict__ out8, float* __restrict__ qscale,
     const unsigned long long* __restrict__ maxes, float* __restrict__ E, float* __restrict__ aux,
-    uint32_t* __restrict__ flags) {
+    uint32_t* __restrict__ flags, int l2) {
   if (blockIdx.x == 0 && threadIdx.x < 2) flags[threadIdx.x] = 0;
   const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (q >= nq) return;
@@ -174,7 +192,7 @@ __global__ __launch_bounds__(256) void query_prep_kernel(
   float sx;
   quantize_row(queries + size_t(q) * d, d, q, KS, prec, out16, out8, s_x, s_h, s_e, sx);
   if ((threadIdx.x & 63) == 0) {
-    E[q] = query_error_bound(s_x, s_h, s_e, prec, 32 * KS, maxes);
+    E[q] = query_error_bound(s_x, s_h, s_e, prec, 32 * KS, maxes, l2 != 0);
     qscale[q] = sx;
     aux[(q >> 4) * 32 + 16 + (q & 15)] = sx;
   }
@@ -264,7 +282,7 @@ struct StageShape {
 // files them per query.
 template <int PREC, int KS, int PASS>
 __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
-    const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
+    const void* __restrict__ xfrag, const float* __restrict__ xscale, const float* __restrict__ xhn, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t tstride,
     uint32_t n_space, uint32_t qsplit, uint32_t* __restrict__ tilemin, uint64_t* __restrict__ emit,
     uint32_t wave_cap, uint32_t* __restrict__ emit_count) {
@@ -300,7 +318,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   for (int f = 0; f < RF; ++f)
 #pragma unroll
     for (int s = 0; s < KS; ++s) b[f][s] = (rb0 + f < nrb) ? xf[(size_t(rb0 + f) * KS + s) * 64 + lane] : FT{};
-  float xs[RF];
+  float xs[RF], hn[RF];  // FP8 row scale; |x|^2 / 2 for L2 (xhn), 0 for inner product
   bool ok[RF];
   uint64_t okm[RF];  // wave masks of the rows inside the base
 #pragma unroll
@@ -309,6 +327,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     ok[f] = row < n;
     okm[f] = ballot(ok[f]);
     xs[f] = (PREC == 2 && ok[f]) ? xscale[row] : 1.f;
+    hn[f] = (xhn && ok[f]) ? xhn[row] : 0.f;
   }
 
   // The caller pads the query fragments (and qaux) to whole stages, so every stage load is
@@ -373,7 +392,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
           float m = __builtin_inff();
 #pragma unroll
           for (int f = 0; f < RF; ++f) {
-            const float sc = PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i];
+            const float sc = (PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i]) - hn[f];
             m = fminf(m, ok[f] ? 1.0f - sc : __builtin_inff());
           }
           m = row16_min(m);
@@ -394,7 +413,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
         for (int f = 0; f < RF; ++f)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            sc[f][i] = PREC == 2 ? acc[f][i] * qsc[i] * xs[f] : acc[f][i];
+            sc[f][i] = (PREC == 2 ? acc[f][i] * qsc[i] * xs[f] : acc[f][i]) - hn[f];
             any |= ballot(sc[f][i] >= cut[i]) & okm[f];
           }
         if (any) {
@@ -1099,10 +1118,22 @@ hipError_t launch_norm_max(hipStream_t st, const double* norms, uint32_t r0, uin
 
 hipError_t launch_query_prep(hipStream_t st, const float* queries, uint32_t d, uint32_t nq, uint32_t KS, int prec,
                              void* out, float* qscale, const unsigned long long* maxes, float* E, float* aux,
-                             uint32_t* flags) {
+                             uint32_t* flags, bool l2) {
   hipLaunchKernelGGL(query_prep_kernel, dim3((std::max(nq, 1u) + 3) / 4), dim3(256), 0, st, queries, d, nq, KS, prec,
                      prec == 1 ? static_cast<__bf16*>(out) : nullptr,
-                     prec == 2 ? static_cast<uint8_t*>(out) : nullptr, qscale, maxes, E, aux, flags);
+                     prec == 2 ? static_cast<uint8_t*>(out) : nullptr, qscale, maxes, E, aux, flags, l2 ? 1 : 0);
+  return hipGetLastError();
+}
+
+// hn[r] = |x_r|^2 / 2 in fp32 (from the FP64 norms) for rows [r0, r1): the L2 term of the estimate
+__global__ void half_norm_kernel(const double* __restrict__ norms, uint32_t r0, uint32_t r1, float* __restrict__ hn) {
+  const uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < r1) hn[r] = float(0.5 * norms[size_t(r) * 3]);
+}
+
+hipError_t launch_half_norms(hipStream_t st, const double* norms, uint32_t r0, uint32_t r1, float* hn) {
+  if (r1 <= r0) return hipSuccess;
+  hipLaunchKernelGGL(half_norm_kernel, dim3((r1 - r0 + 255) / 256), dim3(256), 0, st, norms, r0, r1, hn);
   return hipGetLastError();
 }
 
@@ -1202,11 +1233,11 @@ hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
   const size_t lds = 0;  // static: four ring slots and the reduction buffer
   const dim3 grid(a.n_blocks), block(64 * kWaves);
   if (pass == 1)
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn, a.n,
                        a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.qsplit, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   else
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.n,
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn, a.n,
                        a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, 1u, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   return hipGetLastError();
