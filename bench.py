@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--filtered", type=int, default=4096,
                     help="queries of the batched filtered scan measurement (0 to skip)")
     ap.add_argument("--filtered-rows", type=int, default=892_800)
+    ap.add_argument("--lists-tile", default="",
+                    help="fixed tile of the filtered scan, qb,r,chunk,lanes_per_row (default: the autotuner's)")
     ap.add_argument("--serve-rows", type=int, default=2_000_000,
                     help="rows of the serving-loop base (0 to skip the serving measurement)")
     ap.add_argument("--serve-dim", type=int, default=768)
@@ -679,6 +681,10 @@ def main():
         fks, fg = datagen.query_filters(args.filtered, 100, seed=3, ks=(10, 100))
         franges = datagen.query_ranges(fg, goff)
         sizes = np.diff(goff)
+        if args.lists_tile:
+            for name, v in zip(("lists_qb", "lists_r", "lists_chunk", "lists_lanes_per_row"),
+                               (int(t) for t in args.lists_tile.split(","))):
+                idx.set_param(name, v)
         t0 = time.perf_counter()
         fres = idx.search_batch(fq, fks, franges, gids)  # first call: the autotuner picks the tile
         tune_s = time.perf_counter() - t0

@@ -159,11 +159,7 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
       for (int c = 0; c < 4; ++c) acc[j][b][c] = 0.f;
 
   const uint32_t nb16 = D16 > 0 ? uint32_t(D16) : d >> 4;
-#pragma unroll 4
-  for (uint32_t s = 0; s < nb16; ++s) {
-    float4 xv[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) xv[j] = load_step<MODE>(xr[j] + s * STEP + 4 * t, ok[j]);
+  auto fma_step = [&](const float4 (&xv)[R], uint32_t s) __attribute__((always_inline)) {
 #pragma unroll
     for (int b = 0; b < QB; ++b) {
       const float4 qv = *reinterpret_cast<const float4*>(q + b * dp + 16 * s + 4 * t);
@@ -174,6 +170,42 @@ __device__ __forceinline__ void quad_dists(const float* const (&xr)[R], const bo
         acc[j][b][2] = fma_term<L2>(qv.z, xv[j].z, acc[j][b][2]);
         acc[j][b][3] = fma_term<L2>(qv.w, xv[j].w, acc[j][b][3]);
       }
+    }
+  };
+  if constexpr (MODE == kLoadStream && D16 > 0 && D16 % 8 == 0 && QB >= 4) {
+    // Several queries per pass: enough FMAs per step that the compiler no longer hoists later
+    // steps' loads above them. Two register sets of four steps each: the next four steps' loads
+    // are issued before the current four steps' FMAs (past the last step: the last step again,
+    // never read); the scheduling barriers keep that order.
+    float4 xa[4][R], xb[4][R];
+    auto load4 = [&](float4 (&dst)[4][R], uint32_t s0) __attribute__((always_inline)) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t s = min(s0 + i, uint32_t(D16 - 1));
+#pragma unroll
+        for (int j = 0; j < R; ++j) dst[i][j] = load_step<MODE>(xr[j] + s * STEP + 4 * t, ok[j]);
+      }
+    };
+    load4(xa, 0);
+    for (uint32_t s = 0; s < uint32_t(D16); s += 8) {
+      load4(xb, s + 4);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) fma_step(xa[i], s + i);
+      __builtin_amdgcn_sched_barrier(0);
+      load4(xa, s + 8);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) fma_step(xb[i], s + 4 + i);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll 4
+    for (uint32_t s = 0; s < nb16; ++s) {
+      float4 xv[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) xv[j] = load_step<MODE>(xr[j] + s * STEP + 4 * t, ok[j]);
+      fma_step(xv, s);
     }
   }
   if (D16 == 0 && (d & 8)) {  // the 8-dim step adds into acc0 only (distance.h:64-65)
