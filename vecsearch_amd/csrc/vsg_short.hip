@@ -20,10 +20,11 @@
 // Error bound E (inner product; all norms computed in FP64 when the copies are built):
 //   |sum q x - sum q^ x^| <= |q| |x - x^| + |q - q^| |x^|           (exact identity + C-S)
 //   MFMA accumulation (products of bf16 / fp8 are exact in fp32)   <= 2 gamma_d |q^| |x^|
-//   FP8 only: the 16x16x32 FP8 instruction's own sum of its 32      <= 2^-10 |q^| |x^|
-//     products is measured at up to 2^-12.4 of their absolute sum (not the fp32 accuracy the
-//     term above assumes); 2^-10 of sum |q^_i x^_i| <= 2^-10 |q^| |x^| covers it 5x
-//     (tests/gpu_probes/mfma_accuracy.hip, run by the GPU tests)
+//   FP8 only: the FP8 instruction's own sum of its products        <= 2^-10 |q^| |x^|
+//     is measured at up to about 2^-12.3 of their absolute sum, both for the 16x16x32
+//     instruction (widths of 64) and for the 16x16x128 one (every other width): not the fp32
+//     accuracy the term above assumes; 2^-10 of sum |q^_i x^_i| <= 2^-10 |q^| |x^| covers both
+//     (tests/gpu_probes/mfma_accuracy.hip, run by the GPU tests, holds each within 2^-11)
 //   FP8 per-vector scales: two extra fp32 roundings                 <= 4 u |q^| |x^|
 //   reference CPU order vs the exact sum                            <= gamma_m |q| |x|
 //   the final 1 - s roundings on both sides                         <= 4 u (1 + |q| |x|)
@@ -39,15 +40,16 @@
 //   hn rounded to fp32                                              <= u |x|max^2 / 2
 //   the reference's l2_sq order vs the exact sum (halved)           <= gamma_m (|q| + |x|max)^2 / 2
 //   the subtraction of hn and the final 1 - s roundings             <= 4 u (1 + |q| |x| + |x|max^2)
-// The rows-stationary pass subtracts hn from every score (0 for inner product: the same bits);
-// L2 batches use that pass for pass 2 too.
+// The rows-stationary pass's L2 instance subtracts hn from every score; L2 batches use that pass
+// for pass 2 too.
 //
 // Data layouts (built from the row-major base when rows are added):
 //   BF16 copy: for row block rb (16 rows) and k-step s (32 dims) a 1 KB chunk; lane l of the
 //     chunk holds rows rb*16 + l%16, dims 32s + 8*(l/16) .. +7 -- exactly the B operand of
 //     v_mfma_f32_16x16x32_bf16, so a wave loads one chunk with one coalesced 16-byte load.
 //   FP8 copy: the same with e4m3 values (8 bytes per lane, 512 B per chunk) of x / s_x, where
-//     s_x = max|x_i| / 448 is the row's fp32 scale (the B operand of ..._16x16x32_fp8_fp8).
+//     s_x = max|x_i| / 448 is the row's fp32 scale; four consecutive k-steps' pieces of a lane form
+//     its operand of v_mfma_f32_16x16x128_f8f6f4.
 //   Queries of a batch are converted into the same layouts (A operands).
 #include <hip/hip_runtime.h>
 
@@ -229,6 +231,20 @@ __device__ __forceinline__ f32x4m_t mfma<2>(long a, long b, f32x4m_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
 }
 
+// FP8 over 128 dims in one instruction (v_mfma_f32_16x16x128_f8f6f4, twice the rate of four
+// 16x16x32 steps): a lane's 32-byte operand is its 8-byte pieces of four consecutive 32-dim
+// k-steps side by side, for A and B alike, so byte j of lane l pairs query and row dim
+// 32 (4t + j / 8) + 8 (l / 16) + j % 8 of both; the fragment copies keep their 32-dim layout.
+// Formats 0, 0: both operands e4m3; scale operands 0: the unscaled instruction.
+typedef __attribute__((ext_vector_type(8))) int i32x8m_t;
+typedef __attribute__((ext_vector_type(4))) long i64x4m_t;
+__device__ __forceinline__ f32x4m_t mfma_fp8x128(i64x4m_t a, i64x4m_t b, f32x4m_t c) {
+  return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(__builtin_bit_cast(i32x8m_t, a),
+                                                          __builtin_bit_cast(i32x8m_t, b), c, 0, 0, 0, 0, 0, 0);
+}
+// k-steps in fours where the width allows (every FP8 width but 64 dims)
+__host__ __device__ constexpr bool fp8_x128(int prec, int ks) { return prec == 2 && ks % 4 == 0; }
+
 constexpr int kWaves = 8;   // waves per block
 constexpr int kNBuf = 4;    // LDS stages in the ring
 constexpr int kAhead = kNBuf - 1;  // stages in flight ahead of the one being consumed
@@ -280,7 +296,7 @@ struct StageShape {
 // outstanding load, the LDS-DMA prefetch included). emit_count[region] receives the number of
 // entries wanted (more than wave_cap: the region overflowed). scatter_candidates_kernel then
 // files them per query.
-template <int PREC, int KS, int PASS>
+template <int PREC, int KS, int PASS, bool L2>
 __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, const float* __restrict__ xhn, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t tstride,
@@ -318,7 +334,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
   for (int f = 0; f < RF; ++f)
 #pragma unroll
     for (int s = 0; s < KS; ++s) b[f][s] = (rb0 + f < nrb) ? xf[(size_t(rb0 + f) * KS + s) * 64 + lane] : FT{};
-  float xs[RF], hn[RF];  // FP8 row scale; |x|^2 / 2 for L2 (xhn), 0 for inner product
+  float xs[RF], hn[RF];  // FP8 row scale; L2: |x|^2 / 2 (xhn), subtracted from every score
   bool ok[RF];
   uint64_t okm[RF];  // wave masks of the rows inside the base
 #pragma unroll
@@ -327,8 +343,14 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     ok[f] = row < n;
     okm[f] = ballot(ok[f]);
     xs[f] = (PREC == 2 && ok[f]) ? xscale[row] : 1.f;
-    hn[f] = (xhn && ok[f]) ? xhn[row] : 0.f;
+    hn[f] = (L2 && ok[f]) ? xhn[row] : 0.f;
   }
+  // the row's score from its MFMA sum (FP8: times both scales; L2: less |x|^2 / 2)
+  auto score = [&](float v, float qs, int f) -> float {
+    const float s = PREC == 2 ? v * qs * xs[f] : v;
+    if constexpr (L2) return s - hn[f];
+    return s;
+  };
 
   // The caller pads the query fragments (and qaux) to whole stages, so every stage load is
   // in bounds and every wave issues exactly LOADS loads per stage.
@@ -377,11 +399,23 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
       f32x4m_t acc[RF];
 #pragma unroll
       for (int f = 0; f < RF; ++f) acc[f] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
+      if constexpr (fp8_x128(PREC, KS)) {
 #pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const FT a = as[(g * KS + s) * 64 + lane];
+        for (int t = 0; t < KS / 4; ++t) {
+          const FT* ap = as + (g * KS + 4 * t) * 64 + lane;
+          const i64x4m_t a = {ap[0], ap[64], ap[128], ap[192]};
 #pragma unroll
-        for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(a, b[f][s], acc[f]);
+          for (int f = 0; f < RF; ++f)
+            acc[f] = mfma_fp8x128(a, i64x4m_t{b[f][4 * t], b[f][4 * t + 1], b[f][4 * t + 2], b[f][4 * t + 3]},
+                                  acc[f]);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const FT a = as[(g * KS + s) * 64 + lane];
+#pragma unroll
+          for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(a, b[f][s], acc[f]);
+        }
       }
       // value i of acc[f] belongs to query qf*16 + 4*(lane/16) + i and row (rb0+f)*16 + lane%16
       const int q4 = 4 * (lane >> 4);
@@ -392,7 +426,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
           float m = __builtin_inff();
 #pragma unroll
           for (int f = 0; f < RF; ++f) {
-            const float sc = (PREC == 2 ? acc[f][i] * qsc * xs[f] : acc[f][i]) - hn[f];
+            const float sc = score(acc[f][i], qsc, f);
             m = fminf(m, ok[f] ? 1.0f - sc : __builtin_inff());
           }
           m = row16_min(m);
@@ -413,7 +447,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
         for (int f = 0; f < RF; ++f)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            sc[f][i] = (PREC == 2 ? acc[f][i] * qsc[i] * xs[f] : acc[f][i]) - hn[f];
+            sc[f][i] = score(acc[f][i], qsc[i], f);
             any |= ballot(sc[f][i] >= cut[i]) & okm[f];
           }
         if (any) {
@@ -619,11 +653,23 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
 #pragma unroll
         for (int j = 0; j < QW; ++j) acc[j] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
       }
+      if constexpr (fp8_x128(PREC, KS)) {  // FP8 stages hold whole fragments (KH = KS)
 #pragma unroll
-      for (int s = 0; s < KH; ++s) {
-        const FT b = bs[(fi * KH + s) * 64 + lane];
+        for (int t = 0; t < KS / 4; ++t) {
+          const FT* bp = bs + (fi * KS + 4 * t) * 64 + lane;
+          const i64x4m_t b = {bp[0], bp[64], bp[128], bp[192]};
 #pragma unroll
-        for (int j = 0; j < QW; ++j) acc[j] = mfma<PREC>(a[j][part * KH + s], b, acc[j]);
+          for (int j = 0; j < QW; ++j)
+            acc[j] = mfma_fp8x128(i64x4m_t{a[j][4 * t], a[j][4 * t + 1], a[j][4 * t + 2], a[j][4 * t + 3]}, b,
+                                  acc[j]);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < KH; ++s) {
+          const FT b = bs[(fi * KH + s) * 64 + lane];
+#pragma unroll
+          for (int j = 0; j < QW; ++j) acc[j] = mfma<PREC>(a[j][part * KH + s], b, acc[j]);
+        }
       }
       if (part + 1 < HS) continue;  // the fragment's second half comes with the next stage
       // value i of acc[j] belongs to query (fragment of j) * 16 + q4 + i and row rf * 16 + lane % 16
@@ -1228,19 +1274,24 @@ hipError_t launch_zero_u32(hipStream_t st, uint32_t* p, size_t n) {
 }
 
 namespace {
-template <int PREC, int KS>
-hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
+template <int PREC, int KS, bool L2>
+hipError_t launch_mfma_ksm(int pass, const ShortScanArgs& a) {
   const size_t lds = 0;  // static: four ring slots and the reduction buffer
   const dim3 grid(a.n_blocks), block(64 * kWaves);
   if (pass == 1)
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.qsplit, a.tilemin, a.emit, a.wave_cap,
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 1, L2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn,
+                       a.n, a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, a.qsplit, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   else
-    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn, a.n,
-                       a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, 1u, a.tilemin, a.emit, a.wave_cap,
+    hipLaunchKernelGGL((mfma_scan_kernel<PREC, KS, 2, L2>), grid, block, lds, a.st, a.xfrag, a.xscale, a.xhn,
+                       a.n, a.qfrag, a.qaux, a.nq, a.tstride, a.n_space, 1u, a.tilemin, a.emit, a.wave_cap,
                        a.emit_count);
   return hipGetLastError();
+}
+// L2 (the rows' |x|^2 / 2 given) or inner product
+template <int PREC, int KS>
+hipError_t launch_mfma_ks(int pass, const ShortScanArgs& a) {
+  return a.xhn ? launch_mfma_ksm<PREC, KS, true>(pass, a) : launch_mfma_ksm<PREC, KS, false>(pass, a);
 }
 
 template <int PREC>
