@@ -669,7 +669,8 @@ int short_reserve(vsg_index* h, Workspace& w, size_t nq, size_t k, int p, ShortP
   if ((e = w.s_qscale.reserve(qpad)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_E.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_thr.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
-  if ((e = w.s_qaux.reserve(qpad / 16 * 32 + vsg::short_aux_pad())) != hipSuccess) return fail_hip("workspace", e);
+  if ((e = w.s_qaux.reserve(qpad / 16 * size_t(vsg::short_aux_floats()) + vsg::short_aux_pad())) != hipSuccess)
+    return fail_hip("workspace", e);
   if ((e = w.s_tilemin.reserve(round * n_tiles)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_count.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_flags.reserve(round)) != hipSuccess) return fail_hip("workspace", e);

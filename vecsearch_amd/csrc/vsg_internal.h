@@ -38,7 +38,7 @@ struct ShortScanArgs {
   const float* xhn = nullptr;  // L2: |x|^2 / 2 per row (the rows-stationary pass subtracts it)
   uint32_t n;
   const void* qfrag;
-  const float* qaux;  // per query fragment: 16 thresholds, then 16 scales (query_aux_kernel)
+  const float* qaux;  // per query fragment: 16 score cuts, 16 scales, 16 folded FP8 cuts
   uint32_t nq;
   uint32_t n_blocks;  // one block per visited tile
   uint32_t n_space;   // visited tiles: pass 1 ceil(n_tiles / tstride), pass 2 n_tiles
@@ -68,6 +68,7 @@ int short_cand_cap();
 int short_tile_rows_for(int prec, int ks);
 int short_group(int prec, int ks);
 int short_aux_pad();
+int short_aux_floats();  // floats of a query fragment's aux block
 hipError_t launch_quantize(hipStream_t st, const float* src, uint32_t ld, uint32_t d, uint32_t r0,
                            uint32_t r1, uint32_t KS, int prec, void* out, float* scale,
                            double* norms);

@@ -154,6 +154,10 @@ __global__ void norm_max_kernel(const double* __restrict__ norms, uint32_t r0, u
   }
 }
 
+// The aux block of a query fragment (16 queries), kAux floats: [0, 16) the pass-2 score cuts,
+// [16, 32) the FP8 query scales (1 for BF16), [32, 48) the folded FP8 cuts of pass 2 (below).
+constexpr int kAux = 48;
+
 // Per-query error bound E (see the header) from the query norms and the base maxima.
 __device__ __forceinline__ float query_error_bound(double qx2, double qh2, double qe2, int prec, uint32_t d,
                                                    const unsigned long long* __restrict__ maxes, bool l2) {
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(256) void query_prep_kernel(
   if ((threadIdx.x & 63) == 0) {
     E[q] = query_error_bound(s_x, s_h, s_e, prec, 32 * KS, maxes, l2 != 0);
     qscale[q] = sx;
-    aux[(q >> 4) * 32 + 16 + (q & 15)] = sx;
+    aux[(q >> 4) * kAux + 16 + (q & 15)] = sx;
   }
 }
 
@@ -278,9 +282,9 @@ struct StageShape {
   static constexpr int CB = Frag<PREC>::kChunkBytes;
   static constexpr int FRAG_BYTES = KS * CB;       // one query fragment, all k-steps
   static constexpr int STAGE_Q = G * FRAG_BYTES;   // query operands of a stage
-  // per query of the stage: threshold, scale (G x 32 floats), rounded up to whole 256-byte
-  // wave loads (64 lanes x 4 bytes); the caller pads qaux by kAuxPad floats past the last stage
-  static constexpr int AUX_BYTES = (G * 32 * 4 + 255) / 256 * 256;
+  // the stage's aux blocks (G x kAux floats), rounded up to whole 256-byte wave loads (64 lanes
+  // x 4 bytes); the caller pads qaux by short_aux_pad() floats past the last stage
+  static constexpr int AUX_BYTES = (G * kAux * 4 + 255) / 256 * 256;
   static constexpr int STAGE_BYTES = STAGE_Q + AUX_BYTES;
   static constexpr int PIECES = STAGE_Q / 1024;    // 1 KB pieces (one wave load each)
   static constexpr int PER_WAVE = PIECES / kWaves; // loads per wave per stage (exact)
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     }
 #pragma unroll
     for (int r = 0; r < SH::AUX_LOADS; ++r)
-      load_lds4(qaux + size_t(st) * G * 32 + r * 64 + lane, dst + SH::STAGE_Q + r * 256);
+      load_lds4(qaux + size_t(st) * G * kAux + r * 64 + lane, dst + SH::STAGE_Q + r * 256);
   };
   auto flush_min = [&](uint32_t st) {  // tile minima of stage st (pass 1), after a barrier
     if (threadIdx.x < G * 16) {
@@ -400,29 +404,59 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
 #pragma unroll
       for (int f = 0; f < RF; ++f) acc[f] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
       if constexpr (fp8_x128(PREC, KS)) {
-#pragma unroll
-        for (int t = 0; t < KS / 4; ++t) {
+        // as below: operands read kAD instructions ahead (four 8-byte reads each, which the
+        // compiler pairs into two ds_read2st64_b64)
+        constexpr int T = KS / 4, kAD = T < 2 ? T : 2;
+        i64x4m_t av[T];
+        auto read_a = [&](int t) {
           const FT* ap = as + (g * KS + 4 * t) * 64 + lane;
-          const i64x4m_t a = {ap[0], ap[64], ap[128], ap[192]};
+          av[t] = i64x4m_t{ap[0], ap[64], ap[128], ap[192]};
+        };
+#pragma unroll
+        for (int t = 0; t < kAD; ++t) read_a(t);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          if (t + kAD < T) read_a(t + kAD);
 #pragma unroll
           for (int f = 0; f < RF; ++f)
-            acc[f] = mfma_fp8x128(a, i64x4m_t{b[f][4 * t], b[f][4 * t + 1], b[f][4 * t + 2], b[f][4 * t + 3]},
+            acc[f] = mfma_fp8x128(av[t], i64x4m_t{b[f][4 * t], b[f][4 * t + 1], b[f][4 * t + 2], b[f][4 * t + 3]},
                                   acc[f]);
         }
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 * kAD, 0);
+#pragma unroll
+        for (int t = 0; t + kAD < T; ++t) {
+          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, RF, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, RF * kAD, 0);
       } else {
+        // A operands read kAD k-steps ahead of their MFMAs: the scheduler is held to that order
+        // (kAD reads, then one read per k-step's RF MFMAs), so each read has kAD steps of MFMA
+        // issue to land in instead of being waited for right away
+        constexpr int kAD = KS < 3 ? KS : 3;
+        FT av[KS];
+#pragma unroll
+        for (int s = 0; s < kAD; ++s) av[s] = as[(g * KS + s) * 64 + lane];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-          const FT a = as[(g * KS + s) * 64 + lane];
+          if (s + kAD < KS) av[s + kAD] = as[(g * KS + s + kAD) * 64 + lane];
 #pragma unroll
-          for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(a, b[f][s], acc[f]);
+          for (int f = 0; f < RF; ++f) acc[f] = mfma<PREC>(av[s], b[f][s], acc[f]);
         }
+        __builtin_amdgcn_sched_group_barrier(0x100, kAD, 0);
+#pragma unroll
+        for (int s = 0; s + kAD < KS; ++s) {
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, RF, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, RF * kAD, 0);
       }
       // value i of acc[f] belongs to query qf*16 + 4*(lane/16) + i and row (rb0+f)*16 + lane%16
       const int q4 = 4 * (lane >> 4);
       if (PASS == 1) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float qsc = PREC == 2 ? aux[g * 32 + 16 + q4 + i] : 1.f;
+          const float qsc = PREC == 2 ? aux[g * kAux + 16 + q4 + i] : 1.f;
           float m = __builtin_inff();
 #pragma unroll
           for (int f = 0; f < RF; ++f) {
@@ -434,36 +468,48 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
         }
       } else {
         // aux holds score cuts: every row with 1 - score <= threshold has score >= cut (padding
-        // queries carry +inf and never emit). One mask test per fragment; the emission path
-        // runs only when some lane hits.
-        float cut[4], qsc[4], sc[RF][4];
+        // queries carry +inf and never emit). Each value's test leaves a lane mask in scalar
+        // registers; the emission path runs only when some mask is not empty, and only for the
+        // values whose mask is not empty.
+        // FP8 inner product tests fl(acc * xs) >= cq, the query's folded cut (one multiply
+        // instead of two); the emission path applies the exact test to the values that pass.
+        constexpr bool kFold = PREC == 2 && !L2;
+        float cut[4], qsc[4], cq[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          cut[i] = aux[g * 32 + q4 + i];
-          qsc[i] = PREC == 2 ? aux[g * 32 + 16 + q4 + i] : 1.f;
+          cut[i] = aux[g * kAux + q4 + i];
+          qsc[i] = PREC == 2 ? aux[g * kAux + 16 + q4 + i] : 1.f;
+          cq[i] = kFold ? aux[g * kAux + 32 + q4 + i] : 0.f;
         }
+        uint64_t hm[RF][4];
         uint64_t any = 0;
 #pragma unroll
         for (int f = 0; f < RF; ++f)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            sc[f][i] = score(acc[f][i], qsc[i], f);
-            any |= ballot(sc[f][i] >= cut[i]) & okm[f];
+            const bool pass = kFold ? acc[f][i] * xs[f] >= cq[i] : score(acc[f][i], qsc[i], f) >= cut[i];
+            hm[f][i] = ballot(pass) & okm[f];
+            any |= hm[f][i];
           }
         if (any) {
 #pragma unroll
           for (int f = 0; f < RF; ++f)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const bool hit = ok[f] && sc[f][i] >= cut[i];
-              const uint64_t m = ballot(hit);
+              uint64_t m = hm[f][i];
               if (m) {
+                const float sc = score(acc[f][i], qsc[i], f);
+                bool hit = (m >> lane) & 1;
+                if constexpr (kFold) {
+                  hit = hit && sc >= cut[i];
+                  m = ballot(hit);
+                }
                 const uint32_t pre =
                     __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
                 const uint32_t at = n_emit + pre;
                 const uint32_t q = qf * 16 + q4 + i;
                 if (hit && at < wave_cap)
-                  wemit[at] = make_key(1.0f - sc[f][i], (q << 6) | uint32_t(f * 16 + (lane & 15)));
+                  wemit[at] = make_key(1.0f - sc, (q << 6) | uint32_t(f * 16 + (lane & 15)));
                 n_emit += uint32_t(__builtin_popcountll(m));
               }
             }
@@ -597,7 +643,7 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
   const uint32_t nqf = (nq + 15) >> 4, q4 = 4 * (lane >> 4);
   const FT* qsrc = reinterpret_cast<const FT*>(qfrag);
   FT a[QW][KS];
-  float cut[QW][4], qsc[QW][4];
+  float cut[QW][4], qsc[QW][4], cq[QW][4];
 #pragma unroll
   for (int j = 0; j < QW; ++j) {
     const uint32_t qf = (qg * kQsWaves + wave) * QW + j;
@@ -606,8 +652,9 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
     for (int s = 0; s < KS; ++s) a[j][s] = qok ? qsrc[(size_t(qf) * KS + s) * 64 + lane] : FT{};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      cut[j][i] = qok ? qaux[qf * 32 + q4 + i] : __builtin_inff();
-      qsc[j][i] = (PREC == 2 && qok) ? qaux[qf * 32 + 16 + q4 + i] : 1.f;
+      cut[j][i] = qok ? qaux[qf * kAux + q4 + i] : __builtin_inff();
+      qsc[j][i] = (PREC == 2 && qok) ? qaux[qf * kAux + 16 + q4 + i] : 1.f;
+      cq[j][i] = (PREC == 2 && qok) ? qaux[qf * kAux + 32 + q4 + i] : __builtin_inff();
     }
   }
   wait_vm<0>();
@@ -678,28 +725,36 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       float xs = 1.f;
       if constexpr (PREC == 2)
         xs = ok ? reinterpret_cast<const float*>(slot(I) + ST + wave * XN * 256)[fi * 16 + (lane & 15)] : 1.f;
-      float sc[QW][4];
+      // one lane mask per value (FP8: the folded test, exact test at emission; see
+      // folded_cut), the emission path only for values whose mask is not empty
+      const uint64_t okm = ballot(ok);
+      uint64_t hm[QW][4];
       uint64_t any = 0;
 #pragma unroll
       for (int j = 0; j < QW; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          sc[j][i] = PREC == 2 ? acc[j][i] * qsc[j][i] * xs : acc[j][i];
-          any |= ballot(ok && sc[j][i] >= cut[j][i]);
+          hm[j][i] = ballot(PREC == 2 ? acc[j][i] * xs >= cq[j][i] : acc[j][i] >= cut[j][i]) & okm;
+          any |= hm[j][i];
         }
       if (any) {
 #pragma unroll
         for (int j = 0; j < QW; ++j)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const bool hit = ok && sc[j][i] >= cut[j][i];
-            const uint64_t m = ballot(hit);
+            uint64_t m = hm[j][i];
             if (m) {
+              const float sc = PREC == 2 ? acc[j][i] * qsc[j][i] * xs : acc[j][i];
+              bool hit = (m >> lane) & 1;
+              if constexpr (PREC == 2) {
+                hit = hit && sc >= cut[j][i];
+                m = ballot(hit);
+              }
               const uint32_t pre =
                   __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
               const uint32_t at = n_emit + pre;
               const uint32_t q = ((qg * kQsWaves + wave) * QW + j) * 16 + q4 + i;
-              if (hit && at < wave_cap) wemit[at] = make_key(1.0f - sc[j][i], (q << kQsShift) | (row - row0));
+              if (hit && at < wave_cap) wemit[at] = make_key(1.0f - sc, (q << kQsShift) | (row - row0));
               n_emit += uint32_t(__builtin_popcountll(m));
             }
           }
@@ -868,12 +923,25 @@ __device__ __forceinline__ float score_cut(float T) {
   return round_down_f(1.0 - t - fabs(t) * 0x1p-22 - 0x1p-60);
 }
 
-// Pass-2 aux block of the query stages: for query q = 16 f + i, aux[f * 32 + i] = its score cut
-// (score_cut of its threshold; +inf for padding queries, which then never emit) and
-// aux[f * 32 + 16 + i] = its scale (1 for padding queries).
+// Folded FP8 cut: a value passes fl(acc * xs) >= folded_cut(cut, qsc) whenever its score
+// fl(fl(acc * qsc) * xs) >= cut (qsc, xs > 0). The three roundings between the two tests
+// (two in the score, one in acc * xs) and the one of t = fl(cut / qsc) move the compared values
+// by at most 4u of their size, so lowering t by 2^-18 of its size (64u) keeps every passing
+// value (the sign of a product survives its rounding). Infinite cuts stay infinite (+inf:
+// nothing passes; -inf: everything does).
+__device__ __forceinline__ float folded_cut(float cut, float qsc) {
+  const float t = cut / qsc;
+  return __builtin_isinf(t) || t != t ? t : t - fabsf(t) * 0x1p-18f;
+}
+
+// Pass-2 aux block of the query stages: for query q = 16 f + i, aux[f * kAux + i] = its score
+// cut (score_cut of its threshold; +inf for padding queries, which then never emit),
+// aux[f * kAux + 16 + i] = its scale (1 for padding queries) and aux[f * kAux + 32 + i] = the
+// folded cut of the two.
 __device__ __forceinline__ void set_pass2_aux(float* aux, uint32_t q, float cut, float sc) {
-  aux[(q >> 4) * 32 + (q & 15)] = cut;
-  aux[(q >> 4) * 32 + 16 + (q & 15)] = sc;
+  aux[(q >> 4) * kAux + (q & 15)] = cut;
+  aux[(q >> 4) * kAux + 16 + (q & 15)] = sc;
+  aux[(q >> 4) * kAux + 32 + (q & 15)] = folded_cut(cut, sc);
 }
 
 // ---- threshold select --------------------------------------------------------------------
@@ -1187,6 +1255,7 @@ hipError_t launch_half_norms(hipStream_t st, const double* norms, uint32_t r0, u
 // and the aux buffer by short_aux_pad() more floats (a stage reads whole 256-byte loads)
 int short_group(int prec, int ks) { return stage_group(prec, ks); }
 int short_aux_pad() { return 64; }
+int short_aux_floats() { return kAux; }
 
 int short_waves() { return kWaves; }
 
