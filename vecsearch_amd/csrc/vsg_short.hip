@@ -397,10 +397,8 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
     const FT* as = reinterpret_cast<const FT*>(stage);
     const float* aux = reinterpret_cast<const float*>(stage + SH::STAGE_Q);
     float* rd = red[st & 1];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const uint32_t qf = st * G + g;
-      f32x4m_t acc[RF];
+    // The MFMA products of fragment g (query fragment st * G + g) against the wave's rows.
+    auto products = [&](int g, f32x4m_t (&acc)[RF]) __attribute__((always_inline)) {
 #pragma unroll
       for (int f = 0; f < RF; ++f) acc[f] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
       if constexpr (fp8_x128(PREC, KS)) {
@@ -450,6 +448,21 @@ __global__ __launch_bounds__(64 * kWaves) void mfma_scan_kernel(
           __builtin_amdgcn_sched_group_barrier(0x008, RF, 0);
         }
         __builtin_amdgcn_sched_group_barrier(0x008, RF * kAD, 0);
+      }
+    };
+    // FP8 pass 2 (registers to spare) issues fragment g + 1's MFMAs before fragment g's
+    // epilogue, so the matrix pipe works through them while the wave runs the compares.
+    constexpr bool kPipe = PASS == 2 && PREC == 2;
+    f32x4m_t accs[2][RF];
+    if constexpr (kPipe) products(0, accs[0]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint32_t qf = st * G + g;
+      f32x4m_t(&acc)[RF] = accs[g & 1];
+      if constexpr (kPipe) {
+        if (g + 1 < G) products(g + 1, accs[(g + 1) & 1]);
+      } else {
+        products(g, acc);
       }
       // value i of acc[f] belongs to query qf*16 + 4*(lane/16) + i and row (rb0+f)*16 + lane%16
       const int q4 = 4 * (lane >> 4);
