@@ -92,7 +92,8 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* queries, uint32_t nq, const uint32_t* ks, uint32_t k,
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
-                          uint32_t* rescored);
+                          uint32_t* rescored, uint32_t* sig, uint32_t* qmap);  // sig: nq, qmap: nq + 7
+                          // (sig, qmap null: queries in batch order)
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.
 struct ListsWorkspace {
