@@ -88,6 +88,7 @@ def parse():
                     help="0 = every core this process may run on (os.sched_getaffinity)")
     ap.add_argument("--p1", type=int, default=100_000, help="rows of the 100k workload (0 to skip)")
     ap.add_argument("--l2", type=int, default=1, help="measure the L2 index on the same base (0 to skip)")
+    ap.add_argument("--q100", type=int, default=1, help="measure both shortlist forms at k = 100 (0 to skip)")
     return ap.parse_args()
 
 
@@ -555,7 +556,7 @@ def main():
 
     # ---- Q at k = 100: both forms at batches 128 / 1,024 / 4,096 ----
     q100 = {}
-    if sb > 0:
+    if sb > 0 and args.q100:
         nchk = 256
         idx.search_device(dq.data_ptr(), nchk, 100, ids100.data_ptr(), dists100.data_ptr(), sptr, "skinny")
         torch.cuda.synchronize()
