@@ -830,9 +830,13 @@ struct RegionGeometry {
 //   offsets per query (one wave), exclusive scan over the slices in place, total -> count[q];
 //   place   per slice, each entry at cand[q * cap + offset + its LDS rank] if below cap.
 // The order of a query's candidates is not fixed; the rescore's result does not depend on it.
+// A slice's block has 16 waves, each walking its own regions, so that a CU keeps 16 regions'
+// loads in flight: each wave's loop is a chain of a load and an LDS atomic.
 constexpr uint32_t kScatterSlices = 1024;
+constexpr uint32_t kScatterThreads = 1024;
+constexpr uint32_t kScatterWaves = kScatterThreads / 64;
 
-__global__ __launch_bounds__(256) void scatter_count_kernel(
+__global__ __launch_bounds__(kScatterThreads) void scatter_count_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
     RegionGeometry geo, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
     uint32_t* __restrict__ slice_cnt, uint32_t* __restrict__ overflow) {
@@ -840,12 +844,12 @@ __global__ __launch_bounds__(256) void scatter_count_kernel(
   __shared__ uint32_t rc[64];
   const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t r0 = s * per, r1 = min(n_regions, r0 + per);
-  for (uint32_t q = tid; q < nq; q += 256) hist[q] = 0;
+  for (uint32_t q = tid; q < nq; q += kScatterThreads) hist[q] = 0;
   for (uint32_t rb = r0; rb < r1; rb += 64) {  // region counts staged 64 at a time
     __syncthreads();
     if (tid < min(64u, r1 - rb)) rc[tid] = emit_count[rb + tid];
     __syncthreads();
-    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += 4) {
+    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += kScatterWaves) {
       const uint32_t c = rc[r - rb];
       if (c > wave_cap) {
         if (lane == 0) atomicAdd(overflow, 1u);
@@ -855,7 +859,7 @@ __global__ __launch_bounds__(256) void scatter_count_kernel(
     }
   }
   __syncthreads();
-  for (uint32_t q = tid; q < nq; q += 256) slice_cnt[size_t(q) * kScatterSlices + s] = hist[q];
+  for (uint32_t q = tid; q < nq; q += kScatterThreads) slice_cnt[size_t(q) * kScatterSlices + s] = hist[q];
 }
 
 // slice_cnt rows have kScatterSlices entries (16-byte aligned); each lane scans 16 of them
@@ -891,7 +895,7 @@ __global__ __launch_bounds__(256) void scatter_offsets_kernel(uint32_t* __restri
   if (lane == 63) count[q] = incl;
 }
 
-__global__ __launch_bounds__(256) void scatter_place_kernel(
+__global__ __launch_bounds__(kScatterThreads) void scatter_place_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
     RegionGeometry geo, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
     const uint32_t* __restrict__ slice_off, uint64_t* __restrict__ cand, uint32_t cap) {
@@ -899,12 +903,12 @@ __global__ __launch_bounds__(256) void scatter_place_kernel(
   __shared__ uint32_t rc[64];
   const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t r0 = s * per, r1 = min(n_regions, r0 + per);
-  for (uint32_t q = tid; q < nq; q += 256) base[q] = slice_off[size_t(q) * kScatterSlices + s];
+  for (uint32_t q = tid; q < nq; q += kScatterThreads) base[q] = slice_off[size_t(q) * kScatterSlices + s];
   for (uint32_t rb = r0; rb < r1; rb += 64) {
     __syncthreads();
     if (tid < min(64u, r1 - rb)) rc[tid] = emit_count[rb + tid];
     __syncthreads();
-    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += 4) {
+    for (uint32_t r = rb + wave; r < min(r1, rb + 64); r += kScatterWaves) {
       const uint32_t c = rc[r - rb];
       if (c > wave_cap) continue;
       for (uint32_t i = lane; i < c; i += 64) {
@@ -1247,23 +1251,68 @@ __global__ __launch_bounds__(256) void cand_signature_kernel(const uint64_t* __r
   if (lane == 0) sig[q] = m;
 }
 
-// One block: sorts (sig, q) of nq <= kOrderMax queries and writes qmap[0, 8 * per) (per =
-// ceil(nq / 8)): block b of the rescore takes sorted position p = (b % 8) * per + b / 8, or no
-// query when p >= nq.
+// One block: orders nq <= kOrderMax queries by signature and writes qmap[0, 8 * per) (per =
+// ceil(nq / 8)): block b of the rescore takes ordered position p = (b % 8) * per + b / 8, or no
+// query when p >= nq. The order is a bucket sort on a multiplicative hash of the signature
+// (kOrderBuckets buckets, one more for queries without candidates): queries with the same
+// signature land in one bucket and so side by side, and different signatures, which crowd the
+// small row ids (a signature is a minimum), spread over the buckets instead of sharing a few.
 constexpr uint32_t kOrderMax = 4096;
+constexpr uint32_t kOrderBuckets = 4096;
 __global__ __launch_bounds__(1024) void rescore_order_kernel(const uint32_t* __restrict__ sig, uint32_t nq,
                                                              uint32_t* __restrict__ qmap) {
-  __shared__ uint64_t keys[kOrderMax];
-  const uint32_t tid = threadIdx.x;
-  uint32_t n2 = 2;
-  while (n2 < nq) n2 <<= 1;
-  for (uint32_t i = tid; i < n2; i += 1024) keys[i] = i < nq ? (uint64_t(sig[i]) << 32) | i : ~uint64_t(0);
+  static_assert(kOrderBuckets == 4096, "12-bit hash");
+  __shared__ uint32_t cnt[kOrderBuckets + 1], sorted[kOrderMax], wsum[16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (uint32_t i = tid; i <= kOrderBuckets; i += 1024) cnt[i] = 0;
   __syncthreads();
-  bitonic_sort_lds<1024>(keys, n2, tid);
+  uint32_t bk[kOrderMax / 1024], rk[kOrderMax / 1024];  // bucket and rank within it, per query
+#pragma unroll
+  for (uint32_t j = 0; j < kOrderMax / 1024; ++j) {
+    const uint32_t q = tid + j * 1024;
+    if (q < nq) {
+      const uint32_t v = sig[q];
+      bk[j] = v != kNoneId ? (v * 0x9E3779B1u) >> 20 : kOrderBuckets;
+      rk[j] = atomicAdd(&cnt[bk[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of cnt[0 .. kOrderBuckets] (4 buckets a thread, the last one by thread 0)
+  uint32_t c[4], t = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c[i] = cnt[tid * 4 + i];
+    t += c[i];
+  }
+  uint32_t incl = t;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= uint32_t(off)) incl += y;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += wsum[w];
+  uint32_t run = before + incl - t;
+  __syncthreads();  // every cnt[] is read before it is overwritten with its offset
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    cnt[tid * 4 + i] = run;
+    run += c[i];
+  }
+  if (tid == 1023) cnt[kOrderBuckets] = run;  // after every bucket: the queries without candidates
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kOrderMax / 1024; ++j) {
+    const uint32_t q = tid + j * 1024;
+    if (q < nq) sorted[cnt[bk[j]] + rk[j]] = q;
+  }
+  __syncthreads();
   const uint32_t per = (nq + 7) / 8;
   for (uint32_t b = tid; b < 8 * per; b += 1024) {
     const uint32_t p = (b % 8) * per + b / 8;
-    qmap[b] = p < nq ? uint32_t(keys[p]) : kNoneId;
+    qmap[b] = p < nq ? sorted[p] : kNoneId;
   }
 }
 
@@ -1340,12 +1389,12 @@ hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
                                std::max(64u, (1u << 20) / a.nq)});
   const uint32_t per = (n_regions + S - 1) / S;
   const size_t lds = size_t(a.nq) * 4;
-  hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap, geo,
-                     n_regions, per, a.nq, S, a.slice, a.overflow);
+  hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(kScatterThreads), lds, a.st, a.emit, a.emit_count,
+                     a.wave_cap, geo, n_regions, per, a.nq, S, a.slice, a.overflow);
   hipLaunchKernelGGL(scatter_offsets_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, a.st, a.slice, S, a.nq,
                      a.count);
-  hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(256), lds, a.st, a.emit, a.emit_count, a.wave_cap, geo,
-                     n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
+  hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(kScatterThreads), lds, a.st, a.emit, a.emit_count,
+                     a.wave_cap, geo, n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
   return hipGetLastError();
 }
 
