@@ -139,7 +139,9 @@ int vsg_search_lists(vsg_index* h, const float* queries, size_t nq, const uint32
  *                          lanes per row 4 or 8 (8: d = 128 / 384 / 768, half the accumulator
  *                          registers per query-row pair)); qb 32 with r 8 and 8 lanes per row
  *                          (d = 128 / 384 / 768): one item per block, its 32 queries sharing
- *                          every row read; 0 = chosen per call (default);
+ *                          every row read; r 1 with 1 lane per row (d a multiple of 32): one
+ *                          row per lane, 8 queries per item whatever qb; 0 = chosen per call
+ *                          (default);
  *   "lists_autotune"       1 (default): with no fixed tile, the first large call of each shape
  *                          class times the candidate tiles on its own inputs and the class keeps
  *                          the fastest; 0: the planner's default tile, and the table is cleared.

@@ -1353,8 +1353,8 @@ int vsg_set_param(vsg_index* h, const char* name, int64_t value) {
     return VSG_OK;
   }
   if (key == "lists_r") {
-    if (value != 0 && value != 2 && value != 4 && value != 8)
-      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 2, 4 or 8");
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)  // 1: the one-lane-per-row tile
+      return fail(VSG_ERR_INVALID_ARGUMENT, "lists_r must be 0 (auto), 1, 2, 4 or 8");
     h->lists_tiles.r = int(value);
     return VSG_OK;
   }
