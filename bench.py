@@ -269,11 +269,21 @@ def serving(args, log):
         lat, el = run(reqs, rate)
         st1 = srv.stats()
         nqs = sum(len(r[0]) for r in reqs)
+        # latency by request class: size (at most 32 queries, or more) x filtered or not
+        by_class = {}
+        for name, sel in (("small_unfiltered", lambda r: len(r[0]) <= 32 and r[2] is None),
+                          ("small_filtered", lambda r: len(r[0]) <= 32 and r[2] is not None),
+                          ("large_unfiltered", lambda r: len(r[0]) > 32 and r[2] is None),
+                          ("large_filtered", lambda r: len(r[0]) > 32 and r[2] is not None)):
+            cl = lat[[i for i, r in enumerate(reqs) if sel(r)]]
+            if len(cl):
+                by_class[name] = {"requests": int(len(cl)), "p50_ms": round(float(np.percentile(cl, 50)), 3),
+                                  "p99_ms": round(float(np.percentile(cl, 99)), 3)}
         rows.append({"offered_requests_per_s": round(rate, 1), "fraction_of_saturation": frac,
                      "requests": nreq, "qps": round(nqs / el, 1),
                      "p50_ms": round(float(np.percentile(lat, 50)), 3),
                      "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                     "batches": st1["batches"] - st0["batches"]})
+                     "batches": st1["batches"] - st0["batches"], "by_class": by_class})
         log(f"  rate {rate:,.0f}/s: {rows[-1]}")
     st = srv.stats()
     srv.close()

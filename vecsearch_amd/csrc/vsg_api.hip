@@ -1557,6 +1557,7 @@ int run_batch(vsg_server* s, ServeSlot& sl, const std::vector<std::shared_ptr<Se
     VSG_HIP(hipMemcpyAsync(w.hdists.p, w.dists.p, nu * ku * sizeof(float), hipMemcpyDeviceToHost, sl.st));
     VSG_HIP(hipStreamSynchronize(sl.st));
     at = 0;
+    bool filtered_follow = false;
     for (const auto& r : batch)
       if (!r->filtered) {
         for (size_t i = 0; i < r->nq; ++i) {  // top-k is the first k of the top-ku
@@ -1564,7 +1565,24 @@ int run_batch(vsg_server* s, ServeSlot& sl, const std::vector<std::shared_ptr<Se
           std::memcpy(r->out_dists + i * r->k, w.hdists.p + (at + i) * ku, r->k * sizeof(float));
         }
         at += r->nq;
+      } else {
+        filtered_follow = true;
       }
+    // the unfiltered requests are answered: release them now instead of after the filtered
+    // launch that follows (their latency is their own part of the batch)
+    if (filtered_follow) {
+      const auto now = Clock::now();
+      {
+        std::lock_guard<std::mutex> lk(s->m);
+        for (const auto& r : batch)
+          if (!r->filtered) {
+            r->status = VSG_OK;
+            r->t_done = now;
+            r->done = true;
+          }
+      }
+      s->cv_done.notify_all();
+    }
   }
   // filtered requests (and a small unfiltered group): one persistent launch
   size_t nl = 0, kl = 0, nid = 0;
@@ -1652,10 +1670,12 @@ void serve_worker(vsg_server* s, ServeSlot* sl) {
     {
       std::lock_guard<std::mutex> lk(s->m);
       for (const auto& r : batch) {
-        r->status = rc;
-        r->err = err;
-        r->t_done = now;
-        r->done = true;
+        if (!r->done) {  // (unfiltered requests of a mixed batch were released by run_batch)
+          r->status = rc;
+          r->err = err;
+          r->t_done = now;
+          r->done = true;
+        }
         s->stats.requests++;
         s->stats.queries += r->nq;
       }
