@@ -902,34 +902,53 @@ int lists_prepare(ListsWorkspace& ws, size_t n, size_t dim, size_t dp, int metri
   // The query blocks of one chunk are consecutive items, so its rows are re-read from cache
   // rather than from HBM. Chunks are dealt to eight queues (one per XCD, for the kernels that
   // use them), each to the queue with the least work so far; the items are then laid out queue
-  // by queue, and a kernel with one queue runs them in that order.
-  std::vector<std::vector<ListItem>> queue(kQueues);
-  std::vector<uint64_t> qwork(kQueues, 0);
+  // by queue, and a kernel with one queue runs them in that order. Chunks are dealt by chunk
+  // index across the groups (every group's first chunk, then every second chunk, ...), so a
+  // query's later chunks mostly start after its earlier ones have published their k-th key
+  // (qthr) and admit few keys, instead of all of a large group's chunks starting from empty
+  // lists at once.
+  struct GroupPlan {
+    uint64_t lb, le;
+    uint32_t direct, nch, qoff0;
+    const std::vector<uint32_t>* qs;
+  };
+  std::vector<GroupPlan> gp;
+  gp.reserve(groups.size());
+  uint32_t max_nch = 0;
   for (const auto& kv : groups) {
     const auto sp = span_of(kv.first);
-    const uint64_t lb = sp.first, le = sp.second;
-    const uint32_t direct = kv.first.first == VSG_ALL_ROWS ? 1u : 0u;
-    const uint32_t nch = uint32_t((le - lb + C - 1) / C);
-    const auto& qs = kv.second;
-    for (uint32_t q : qs) nchunks_of[q] = nch;
-    const uint32_t qoff0 = uint32_t(qperm.size());
-    for (uint32_t q : qs) qperm.push_back(q);
-    for (uint32_t c = 0; c < nch; ++c) {
+    GroupPlan g;
+    g.lb = sp.first;
+    g.le = sp.second;
+    g.direct = kv.first.first == VSG_ALL_ROWS ? 1u : 0u;
+    g.nch = uint32_t((g.le - g.lb + C - 1) / C);
+    g.qs = &kv.second;
+    for (uint32_t q : kv.second) nchunks_of[q] = g.nch;
+    g.qoff0 = uint32_t(qperm.size());
+    for (uint32_t q : kv.second) qperm.push_back(q);
+    max_nch = std::max(max_nch, g.nch);
+    gp.push_back(g);
+  }
+  std::vector<std::vector<ListItem>> queue(kQueues);
+  std::vector<uint64_t> qwork(kQueues, 0);
+  for (uint32_t c = 0; c < max_nch; ++c)
+    for (const GroupPlan& g : gp) {
+      if (c >= g.nch) continue;
+      const auto& qs = *g.qs;
       const uint32_t qi = uint32_t(std::min_element(qwork.begin(), qwork.end()) - qwork.begin());
-      const uint64_t clb = lb + uint64_t(c) * C, cle = std::min<uint64_t>(le, clb + C);
+      const uint64_t clb = g.lb + uint64_t(c) * C, cle = std::min<uint64_t>(g.le, clb + C);
       qwork[qi] += (cle - clb) * qs.size();
       for (size_t b0 = 0; b0 < qs.size(); b0 += qb) {
         ListItem it;
         it.lb = uint32_t(clb);
         it.le = uint32_t(cle);
         it.chunk = c;
-        it.qoff = qoff0 + uint32_t(b0);
+        it.qoff = g.qoff0 + uint32_t(b0);
         it.nq = uint32_t(std::min<size_t>(qb, qs.size() - b0));
-        it.direct = direct;
+        it.direct = g.direct;
         queue[qi].push_back(it);
       }
     }
-  }
   uint32_t bounds[kQueues + 1] = {0};
   for (uint32_t qi = 0; qi < kQueues; ++qi) {
     items.insert(items.end(), queue[qi].begin(), queue[qi].end());
