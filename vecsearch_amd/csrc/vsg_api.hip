@@ -1026,7 +1026,7 @@ int run_lists(vsg_index* h, Workspace& w, hipStream_t st, const float* queries, 
                                     {2, 4, 0, 4},     {8, 4, 1024, 8},  {8, 4, 4096, 8}, {8, 2, 2048, 8},
                                     {16, 2, 2048, 8}, {16, 2, 4096, 8}, {32, 8, 0, 8},   {32, 8, 2048, 8},
                                     {32, 8, 8192, 8}, {8, 1, 0, 1},     {8, 1, 1024, 1}, {8, 1, 2048, 1},
-                                    {8, 1, 4096, 1}};
+                                    {8, 1, 4096, 1},  {8, 1, 8192, 1}};
       vsg::ListsTiles best;
       float best_ms = -1.f;
       for (const auto& c : cand) {

@@ -181,7 +181,7 @@ __device__ __forceinline__ uint32_t lane_slot(uint32_t r, uint32_t j) {
 // of them.)
 template <int NQ, bool L2>
 __device__ __forceinline__ void lane_fma_step(const float4 (&xv)[4], const float* __restrict__ qs,
-                                              const float* __restrict__ qs_next, uint32_t d, float (&acc)[NQ][16],
+                                              const float* __restrict__ qs_next, size_t d, float (&acc)[NQ][16],
                                               float (&qv)[2][32]) {
   static_assert(NQ % 2 == 0, "queries in pairs");
   // an even number of pairs: qv[0] holds this step's first pair on entry (loaded during the
@@ -265,7 +265,10 @@ __device__ __forceinline__ void lane_rows(const float* const (&src)[4], uint8_t*
     float4 xv[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) xv[j] = *reinterpret_cast<const float4*>(cur + lane_slot(uint32_t(lane), uint32_t(j)));
-    lane_fma_step<NQ, L2>(xv, qb + 16 * s, qb + 16 * (s < last ? s + 1 : s), d, acc, qv);
+    // the query row stride as a constant where the width is one (the loads' offsets are then
+    // immediates rather than scalar-register arithmetic)
+    lane_fma_step<NQ, L2>(xv, qb + 16 * s, qb + 16 * (s < last ? s + 1 : s), D16 > 0 ? size_t(D16) * 16 : size_t(d),
+                          acc, qv);
   }
   wait_vm<0>();  // the extra loads past the last step land before the slots are reused
   // the tree of hsum256 on (acc0 + acc1): v_j = a_j + a_{j+8}, w_c = v_c + v_{c+4},
@@ -354,6 +357,11 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
       }
       __syncthreads();  // the round's keys are in LDS
       if (owner && kq > 0) {
+        // a tighter bound that the query's other chunks published since (a relaxed load of a
+        // value that only decreases: any value read is a valid bound)
+        const uint64_t pub = __hip_atomic_load(qthr + myq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t thr0 = pub < thr ? pub : thr;
+        thr = thr0;
         const uint32_t nw = min(uint32_t(kLaneWaves), (w.le - r0 + 63) / 64);  // waves with rows
 #pragma unroll 1
         for (uint32_t c = 0; c < nw; ++c) {
@@ -361,6 +369,9 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
           if (ballot(key < thr) == 0) continue;
           thr = offer64<KP>(lst, thr, key, lane, kq);
         }
+        // publish this chunk's bound as soon as the round tightened it, not only at the item's
+        // end, so that the query's chunks running at the same time admit fewer keys
+        if (lane == 0 && thr < thr0) atomicMin(reinterpret_cast<unsigned long long*>(qthr + myq), thr);
       }
       __syncthreads();  // the keys are read before the next round writes them
     }
