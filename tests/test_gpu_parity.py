@@ -364,8 +364,9 @@ def test_shortlist_ragged_and_tiny_bases(n, nq):
             assert_same(gpu.search(q, k, method=method), oracle.search(x, q, k, "ip", threads=8))
 
 
-@pytest.mark.parametrize("d", [100, 128, 256, 768])
+@pytest.mark.parametrize("d", [32, 64, 100, 128, 256, 768])
 def test_shortlist_other_dims(d):
+    # widths of 64 and below take FP8's 32-dim MFMA instruction, the others its 128-dim one
     x = dg.mixture(66_000, d, seed=42)
     q = dg.mixture(70, d, seed=43)
     gpu = vs.GpuBruteForceIndex(d, "ip")
