@@ -313,6 +313,13 @@ __device__ __forceinline__ void load_lds16(const void* g, void* lds_base) {
 __device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
   __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 4, 0, 0);
 }
+// The same, non-temporal (cache policy nt): for data streamed once, such as a pass over the base.
+__device__ __forceinline__ void load_lds16_nt(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 16, 0, 2);
+}
+__device__ __forceinline__ void load_lds4_nt(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((glb_vptr)g, (lds_vptr)lds_base, 4, 0, 2);
+}
 // s_waitcnt vmcnt(N), N < 64 (gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt
 // left at their maxima so only vector-memory operations are waited for)
 template <int N>

@@ -616,7 +616,9 @@ struct QsShape {
   static_assert(HS == 1 || (SF == 1 && KS % HS == 0), "half stages hold one fragment");
 };
 
-template <int PREC, int KS>
+// NT: the rows stream through non-temporal loads (a single query group reads them once;
+// several groups re-read them from L2)
+template <int PREC, int KS, bool NT>
 __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kernel(
     const void* __restrict__ xfrag, const float* __restrict__ xscale, uint32_t n,
     const void* __restrict__ qfrag, const float* __restrict__ qaux, uint32_t nq, uint32_t rows_per_block,
@@ -692,12 +694,18 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
       const uint32_t at = part * ST + piece * 1024, fi = at / FB;  // byte of the stage's fragments
       uint32_t rf = rf0 + fg * SF + fi;
       if (rf >= rf1) rf = rf0;  // past the block's rows: a fragment that is never read
-      load_lds16(xsrc + size_t(rf) * FB + (at - fi * FB) + lane * 16, dst + piece * 1024);
+      if constexpr (NT)
+        load_lds16_nt(xsrc + size_t(rf) * FB + (at - fi * FB) + lane * 16, dst + piece * 1024);
+      else
+        load_lds16(xsrc + size_t(rf) * FB + (at - fi * FB) + lane * 16, dst + piece * 1024);
     }
 #pragma unroll
     for (int r = 0; r < XN; ++r) {  // rows past the stage's SF * 16: loaded, never read
       const uint32_t row = (rf0 + fg * SF) * 16 + uint32_t(r * 64 + lane);
-      load_lds4(xscale + (row < n ? row : n - 1), dst + ST + (wave * XN + r) * 256);
+      if constexpr (NT)
+        load_lds4_nt(xscale + (row < n ? row : n - 1), dst + ST + (wave * XN + r) * 256);
+      else
+        load_lds4(xscale + (row < n ? row : n - 1), dst + ST + (wave * XN + r) * 256);
     }
   };
   uint32_t n_emit = 0;  // wave-uniform
@@ -806,6 +814,149 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
     if (++st == nst) break;
   }
   wait_vm<0>();  // the refetches land before the block's LDS is released
+  if (lane == 0) emit_count[region] = n_emit;
+}
+
+// ---- pass 2 for small batches, BF16: queries stationary, k-split, rows through registers ----
+// The LDS-DMA ring of mfma_qs_kernel holds at most three 12 KB stages in flight per block and
+// three blocks per CU, and streams at about 0.76 of HBM even without its products. Here a block
+// of 4 waves serves one query group (8 fragments, 128 queries) over a row range with no staging:
+// wave w holds the A operands of all 8 fragments for k-steps [w KQ, (w + 1) KQ) (KQ = KS / 4) and
+// loads its k-steps of every row fragment straight into registers (one coalesced 1 KB chunk per
+// k-step), kQkDepth fragments ahead, so the block reads each row fragment once and two blocks a
+// CU keep 2 x 4 x kQkDepth x KQ KB in flight. A wave's MFMAs give partial sums over its k-steps
+// for the group's 128 queries against the fragment's 16 rows; the partials go to LDS (two
+// buffers), and after one barrier wave w adds the four waves' partials of fragments 2w and
+// 2w + 1 (in wave order) and runs mfma_qs_kernel's emission for them: the same fragments, the
+// same regions and the same entries as that kernel. The products are summed in another order
+// than in the other passes, so an estimate may differ from theirs in its last bits; each is
+// within the bound E (any fp32 order of the d products' sum is within 2 gamma_d |q^| |x^|).
+constexpr int kQkDepth = 6;
+// The queries-stationary passes stream their rows with non-temporal loads when one query group
+// reads them once (at 1M x 384, batch 128: BF16 0.151 -> 0.127 ms, FP8 0.086 -> 0.071 ms); with
+// two to four groups the later groups re-read the rows from L2 and plain loads are as fast or
+// faster (BF16 at 512 queries: 0.332 against 0.360 ms).
+constexpr uint32_t kQsNtGroups = 1;
+__host__ __device__ constexpr bool qk_supported(int prec, int ks) {
+  return prec == 1 && (ks == 4 || ks == 8 || ks == 12);
+}
+
+template <int KS>
+__global__ __launch_bounds__(64 * kQsWaves, 2) void mfma_qk_kernel(
+    const void* __restrict__ xfrag, uint32_t n, const void* __restrict__ qfrag, const float* __restrict__ qaux,
+    uint32_t nq, uint32_t rows_per_block, uint32_t nbx, uint32_t ngroups, uint64_t* __restrict__ emit,
+    uint32_t wave_cap, uint32_t* __restrict__ emit_count) {
+  static_assert(KS % kQsWaves == 0, "k-steps split over the waves");
+  constexpr int KQ = KS / kQsWaves;  // k-steps per wave
+  constexpr int QF = kQsWaves * 2;   // query fragments per group, two emitted by each wave
+  constexpr int D = kQkDepth;
+  __shared__ __attribute__((aligned(16))) f32x4m_t part[2][kQsWaves][QF][64];  // 64 KB
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // block (bx, qg) as in mfma_qs_kernel: the query groups of one row range on one XCD
+  const uint32_t bx = blockIdx.x / (8 * ngroups) * 8 + blockIdx.x % 8, qg = blockIdx.x / 8 % ngroups;
+  const uint32_t row0 = bx * rows_per_block;
+  const uint32_t region = (qg * nbx + bx) * kQsWaves + wave;
+  uint64_t* wemit = emit + size_t(region) * wave_cap;
+  if (row0 >= n) {  // the whole block: no rows
+    if (lane == 0) emit_count[region] = 0;
+    return;
+  }
+  const uint32_t nrf = (n + 15) >> 4;
+  const uint32_t rf0 = row0 >> 4, rf1 = min(nrf, rf0 + rows_per_block / 16);
+  const uint32_t nf = rf1 - rf0;  // row fragments of the block (>= 1)
+
+  // A operands: the group's fragments, this wave's k-steps (fragments past the batch: zero)
+  const uint32_t nqf = (nq + 15) >> 4, q4 = 4 * (lane >> 4);
+  const bf16x8_t* qsrc = reinterpret_cast<const bf16x8_t*>(qfrag);
+  bf16x8_t a[QF][KQ];
+#pragma unroll
+  for (int j = 0; j < QF; ++j) {
+    const uint32_t qf = qg * QF + j;
+#pragma unroll
+    for (int s = 0; s < KQ; ++s)
+      a[j][s] = qf < nqf ? qsrc[(size_t(qf) * KS + wave * KQ + s) * 64 + lane] : bf16x8_t{};
+  }
+  // score cuts of the two fragments this wave emits (padding queries: +inf, never emit)
+  float cut[2][4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t qf = qg * QF + 2 * wave + j;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cut[j][i] = qf < nqf ? qaux[qf * kAux + q4 + i] : __builtin_inff();
+  }
+
+  // this wave's k-steps of row fragment f (past the block's last fragment: the last one again,
+  // loaded and never used)
+  const bf16x8_t* xsrc = reinterpret_cast<const bf16x8_t*>(xfrag);
+  bf16x8_t b[D][KQ];
+  auto load = [&](int slot, uint32_t f) __attribute__((always_inline)) {
+    const uint32_t rf = rf0 + (f < nf ? f : nf - 1);
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) b[slot][s] = __builtin_nontemporal_load(&xsrc[(size_t(rf) * KS + wave * KQ + s) * 64 + lane]);
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) load(u, uint32_t(u));
+
+  uint32_t n_emit = 0;  // wave-uniform
+  for (uint32_t f0 = 0; f0 < nf; f0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const uint32_t f = f0 + uint32_t(u);
+      if (f >= nf) break;  // block-uniform
+      f32x4m_t acc[QF];
+#pragma unroll
+      for (int j = 0; j < QF; ++j) acc[j] = f32x4m_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < KQ; ++s)
+#pragma unroll
+        for (int j = 0; j < QF; ++j) acc[j] = mfma<1>(a[j][s], b[u][s], acc[j]);
+      load(u, f + D);  // the slot's next fragment
+      f32x4m_t* pw = &part[f & 1][wave][0][0];
+#pragma unroll
+      for (int j = 0; j < QF; ++j) pw[j * 64 + lane] = acc[j];
+      // every wave's partials of fragment f are in LDS; the buffer written next (f + 1) was
+      // last read for fragment f - 1, which every wave finished before reaching this barrier
+      __syncthreads();
+      // value i of t[j] belongs to query (qg * QF + 2 wave + j) * 16 + q4 + i and row rf * 16 + lane % 16
+      const uint32_t rf = rf0 + f;
+      const uint32_t row = rf * 16 + (lane & 15);
+      const uint64_t okm = ballot(row < n);
+      f32x4m_t t[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        t[j] = part[f & 1][0][2 * wave + j][lane];
+#pragma unroll
+        for (int v = 1; v < kQsWaves; ++v) t[j] += part[f & 1][v][2 * wave + j][lane];
+      }
+      uint64_t hm[2][4];
+      uint64_t any = 0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          hm[j][i] = ballot(t[j][i] >= cut[j][i]) & okm;
+          any |= hm[j][i];
+        }
+      if (any) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint64_t m = hm[j][i];
+            if (m) {
+              const bool hit = (m >> lane) & 1;
+              const uint32_t pre =
+                  __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+              const uint32_t at = n_emit + pre;
+              const uint32_t q = (qg * QF + 2 * wave + j) * 16 + q4 + i;
+              if (hit && at < wave_cap) wemit[at] = make_key(1.0f - t[j][i], (q << kQsShift) | (row - row0));
+              n_emit += uint32_t(__builtin_popcountll(m));
+            }
+          }
+      }
+    }
+  }
+  wait_vm<0>();  // the loads past the last fragment land before the wave ends
   if (lane == 0) emit_count[region] = n_emit;
 }
 
@@ -1403,13 +1554,33 @@ size_t short_scatter_slices() { return kScatterSlices; }
 namespace {
 template <int PREC, int KS>
 hipError_t launch_qs_t(const ShortScanArgs& a) {
-  hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS>), dim3(a.qs_blocks * a.qs_groups), dim3(64 * kQsWaves), 0, a.st,
+  if (a.qs_groups <= kQsNtGroups) {
+    hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS, true>), dim3(a.qs_blocks * a.qs_groups), dim3(64 * kQsWaves), 0,
+                       a.st, a.xfrag, a.xscale, a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.qs_blocks, a.qs_groups,
+                       a.emit, a.wave_cap, a.emit_count);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((mfma_qs_kernel<PREC, KS, false>), dim3(a.qs_blocks * a.qs_groups), dim3(64 * kQsWaves), 0, a.st,
                      a.xfrag, a.xscale, a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.qs_blocks, a.qs_groups, a.emit,
                      a.wave_cap, a.emit_count);
   return hipGetLastError();
 }
+template <int KS>
+hipError_t launch_qk_t(const ShortScanArgs& a) {
+  hipLaunchKernelGGL((mfma_qk_kernel<KS>), dim3(a.qs_blocks * a.qs_groups), dim3(64 * kQsWaves), 0, a.st, a.xfrag,
+                     a.n, a.qfrag, a.qaux, a.nq, a.qs_rows, a.qs_blocks, a.qs_groups, a.emit, a.wave_cap,
+                     a.emit_count);
+  return hipGetLastError();
+}
 template <int PREC>
 hipError_t launch_qs_p(const ShortScanArgs& a) {
+  if constexpr (PREC == 1) {  // BF16, one query group, widths whose k-steps split over 4 waves
+    if (a.qs_groups == 1) {
+      if (a.KS == 4) return launch_qk_t<4>(a);
+      if (a.KS == 8) return launch_qk_t<8>(a);
+      if (a.KS == 12) return launch_qk_t<12>(a);
+    }
+  }
   switch (a.KS) {
     case 2: return launch_qs_t<PREC, 2>(a);
     case 4: return launch_qs_t<PREC, 4>(a);
@@ -1427,10 +1598,12 @@ bool short_qs_plan(int prec, int ks, uint32_t n, uint32_t nq, int num_cus, uint3
   const uint32_t qpg = uint32_t(kQsWaves * qs_qw(prec, ks) * 16);  // queries per query group
   const uint32_t g = (nq + qpg - 1) / qpg;
   if (nq == 0 || g > kQsMaxGroups || n == 0 || num_cus <= 0) return false;
-  // blocks for all groups: about qs_bpc per CU (one pass over the grid), rows a multiple of
-  // the stage's rows, row ranges a multiple of 8 (the XCD interleave of the groups)
+  // blocks for all groups: about qs_bpc per CU (one pass over the grid; the k-split kernel,
+  // which takes a single BF16 group, runs two), rows a multiple of the stage's rows, row ranges
+  // a multiple of 8 (the XCD interleave of the groups)
   const uint32_t unit = 16u * uint32_t(qs_sf(prec, ks));
-  const uint32_t nb = std::max(8u, uint32_t(num_cus) * uint32_t(qs_bpc(prec, ks)) / g);
+  const uint32_t bpc = qk_supported(prec, ks) && g == 1 ? 2u : uint32_t(qs_bpc(prec, ks));
+  const uint32_t nb = std::max(8u, uint32_t(num_cus) * bpc / g);
   uint32_t r = (n + nb - 1) / nb;
   r = std::min(kQsMaxRows, (r + unit - 1) / unit * unit);
   *rows = r;
