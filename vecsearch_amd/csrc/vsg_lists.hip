@@ -165,7 +165,10 @@ constexpr uint32_t kLaneRound = 64 * kLaneWaves;                    // rows per 
 constexpr size_t kLaneKeyBytes = size_t(kLaneQ) * kLaneRound * 8;  // 32 KB
 constexpr size_t kLaneStage = 4096;                                // one step of 64 rows
 constexpr int kLaneSlots = 3;                                     // ring slots per wave
-constexpr size_t kLaneLds = kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage + 16;
+// two key buffers (rounds alternate; the item index is passed in the second, which no round
+// writes before every wave has read it) and the waves' rings: 160 KB, a CU's whole LDS
+constexpr size_t kLaneLds = 2 * kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage;
+static_assert(kLaneLds <= 160 * 1024, "one workgroup a CU");
 
 // Byte offset of 16-byte chunk j of row r (0..63) in a staged step.
 __device__ __forceinline__ uint32_t lane_slot(uint32_t r, uint32_t j) {
@@ -292,10 +295,11 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
     uint64_t* __restrict__ part, uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint64_t* key_s = reinterpret_cast<uint64_t*>(lds_raw);                // kLaneQ x kLaneRound
-  uint8_t* stage = lds_raw + kLaneKeyBytes + size_t(wave) * kLaneSlots * kLaneStage;  // this wave's ring
-  uint32_t* item_s =
-      reinterpret_cast<uint32_t*>(lds_raw + kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage);
+  uint64_t* keys2 = reinterpret_cast<uint64_t*>(lds_raw);  // 2 x kLaneQ x kLaneRound
+  uint8_t* stage = lds_raw + 2 * kLaneKeyBytes + size_t(wave) * kLaneSlots * kLaneStage;  // this wave's ring
+  // the item index: in the second key buffer, first written by the item's second round, after
+  // the first round's barrier (every wave has read the index by then)
+  uint32_t* item_s = reinterpret_cast<uint32_t*>(lds_raw + kLaneKeyBytes);
   const uint32_t nd16 = D16 > 0 ? uint32_t(D16) : d >> 4;
   const uint32_t* bounds = counter + kBoundsAt;
 
@@ -331,7 +335,13 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
     WaveList<KP> lst;
     lst.init();
 
-    for (uint32_t r0 = w.lb; r0 < w.le; r0 += kLaneRound) {
+    uint32_t par = 0;  // key buffer of the round
+    for (uint32_t r0 = w.lb; r0 < w.le; r0 += kLaneRound, par ^= 1u) {
+      // Round r writes buffer r % 2 and its owners then offer from it. Round r + 1 writes the
+      // other buffer, last read by round r - 1's offers, which every wave finished before round
+      // r's barrier: one barrier a round, and a wave with few keys to offer goes on to the next
+      // round's rows while the others still offer.
+      uint64_t* key_s = keys2 + size_t(par) * (kLaneKeyBytes / 8);
       const uint32_t my0 = r0 + 64u * uint32_t(wave);
       const uint32_t idx = my0 + uint32_t(lane);
       const bool ok = idx < w.le;
@@ -373,7 +383,6 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
         // end, so that the query's chunks running at the same time admit fewer keys
         if (lane == 0 && thr < thr0) atomicMin(reinterpret_cast<unsigned long long*>(qthr + myq), thr);
       }
-      __syncthreads();  // the keys are read before the next round writes them
     }
 
     if (owner) {
