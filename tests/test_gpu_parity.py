@@ -309,11 +309,12 @@ def test_shortlist_matches_oracle(mixture_70k, method, k):
 
 
 @pytest.mark.parametrize("method", ["bf16", "fp8"])
-@pytest.mark.parametrize("nq", [1, 16, 100, 128])
+@pytest.mark.parametrize("nq", [1, 4, 16, 64, 100, 128])
 @pytest.mark.parametrize("k", [10, 100])
 def test_shortlist_small_batches(mixture_70k, method, nq, k):
     # up to 128 queries (d <= 384) take the queries-stationary pass 2: same bits as the oracle,
-    # and as the same queries inside a larger batch (rows-stationary pass 2)
+    # and as the same queries inside a larger batch (rows-stationary pass 2); up to 64 queries
+    # also rescore each query in several blocks (64, 16, 4: 4, 16, 32 shares) and merge them
     x, q, gpu = mixture_70k
     got = gpu.search(q[:nq], k, method=method)
     assert_same(got, oracle.search(x, q[:nq], k, "ip", threads=8))

@@ -146,6 +146,7 @@ struct Workspace {
   DevBuf<uint8_t> s_qfrag;
   DevBuf<float> s_qscale, s_E, s_thr, s_qaux;
   DevBuf<uint32_t> s_tilemin, s_count, s_flags, s_rescored, s_nover, s_sig, s_qmap;
+  DevBuf<uint64_t> s_rpart;  // rescore shares' first-k keys (small batches)
   DevBuf<uint64_t> s_cand;
   DevBuf<uint64_t> s_emit;
   DevBuf<uint32_t> s_emit_count, s_slice;
@@ -181,6 +182,7 @@ struct Workspace {
     s_nover.release();
     s_sig.release();
     s_qmap.release();
+    s_rpart.release();
     s_cand.release();
     s_emit.release();
     s_emit_count.release();
@@ -680,6 +682,10 @@ int short_reserve(vsg_index* h, Workspace& w, size_t nq, size_t k, int p, ShortP
   if ((e = w.s_nover.reserve(2)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_sig.reserve(round)) != hipSuccess) return fail_hip("workspace", e);
   if ((e = w.s_qmap.reserve(round + 7)) != hipSuccess) return fail_hip("workspace", e);
+  // the rescore shares of a small batch (a round of nq queries takes at most max(nq, CUs)
+  // shares in all)
+  if ((e = w.s_rpart.reserve(std::max<size_t>(round, size_t(std::max(1, h->num_cus))) * k)) != hipSuccess)
+    return fail_hip("workspace", e);
   if ((e = w.s_cand.reserve(round * cap)) != hipSuccess) return fail_hip("workspace", e);
   // Pass 2 runs queries-stationary when the batch fits one block's registers (one round),
   // else rows-stationary over 512-row tiles; L2 always rows-stationary (the pass that carries
@@ -815,7 +821,8 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
       VSG_HIP(vsg::launch_rescore(st, h->metric == VSG_METRIC_L2, h->base, uint32_t(h->dp), uint32_t(h->dim),
                                   qr, rq, nullptr, uint32_t(k), w.s_E.p, w.s_cand.p, w.s_count.p, ids,
                                   dists, uint32_t(k), w.s_flags.p, w.s_nover.p, w.s_rescored.p,
-                                  p == 2 ? w.s_sig.p : nullptr, p == 2 ? w.s_qmap.p : nullptr));
+                                  p == 2 ? w.s_sig.p : nullptr, p == 2 ? w.s_qmap.p : nullptr,
+                                  vsg::short_rescore_split(rq, uint32_t(k), h->num_cus), w.s_rpart.p));
       ts.launches++;
     }
     if (dev_fallback) {

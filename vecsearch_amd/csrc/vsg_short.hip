@@ -1247,6 +1247,55 @@ __device__ void bitonic_sort_lds(uint64_t* buf, uint32_t n2, uint32_t tid) {
     }
 }
 
+// Writes the first kq of the r unique keys in buf[0, r) (r <= CAP) in ascending order through
+// put(rank, key): by counting for up to 256 keys; else through a radix select of the kq-th
+// smallest distance bits T, which leaves the keys with distance bits <= T (every key of rank < kq,
+// all others being larger) to be ranked among themselves; a bitonic sort only where ties put more
+// than kTopCap keys at or below T. Every thread of the block calls it.
+template <int NT, class Put>
+__device__ __forceinline__ void rank_first(uint64_t* buf, uint32_t r, uint32_t kq, uint32_t* hist, uint32_t* sel, uint64_t* top,
+                           uint32_t* ntop, uint32_t tid, Put put) {
+  bool sorted = false;
+  if (r > 256 && kq > 0) {
+    const uint32_t th = block_select_hi32<NT>(buf, r, min(kq, r), hist, sel, tid);
+    if (tid == 0) *ntop = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < r; i += NT) {
+      const uint64_t v = buf[i];
+      if (uint32_t(v >> 32) <= th) {
+        const uint32_t p = atomicAdd(ntop, 1u);
+        if (p < uint32_t(kTopCap)) top[p] = v;
+      }
+    }
+    __syncthreads();
+    const uint32_t nt = *ntop;
+    if (nt <= uint32_t(kTopCap)) {
+      for (uint32_t i = tid; i < nt; i += NT) {
+        const uint64_t v = top[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < nt; ++j) rank += top[j] < v;
+        if (rank < kq) put(rank, v);
+      }
+      sorted = true;
+    }
+  }
+  if (!sorted && r <= uint32_t(NT)) {
+    for (uint32_t i = tid; i < r; i += NT) {
+      const uint64_t v = buf[i];
+      uint32_t rank = 0;
+      for (uint32_t j = 0; j < r; ++j) rank += buf[j] < v;
+      if (rank < kq) put(rank, v);
+    }
+  } else if (!sorted) {
+    uint32_t m2 = 2 * NT;  // <= CAP: more than NT keys came from at most CAP
+    while (m2 < r) m2 <<= 1;
+    for (uint32_t i = r + tid; i < m2; i += NT) buf[i] = kPadKey;
+    __syncthreads();
+    bitonic_sort_lds<NT>(buf, m2, tid);
+    for (uint32_t i = tid; i < kq && i < r; i += NT) put(i, buf[i]);
+  }
+}
+
 // Handles the queries with cmin <= count <= CAP; with final_pass set, flags those above CAP.
 // NT threads per block: 1024 where the batch has few queries (one block each), so that a
 // query's candidate rows are read by 16 waves at once instead of 4.
@@ -1370,6 +1419,125 @@ __global__ __launch_bounds__(NT) void rescore_kernel(
     }
   }
   for (uint32_t i = r + tid; i < kq; i += NT) {
+    oi[i] = kNoneId;
+    od[i] = __builtin_inff();
+  }
+}
+
+// The rescore of a batch of at most a quarter as many queries as CUs, with `split` blocks a
+// query: block s rescores the candidates i with i % split == s (after the same select and cut as
+// rescore_kernel, over all of the query's candidates) and writes its first kq exact keys to
+// rpart (kPadKey past its own); rescore_merge_kernel then ranks the split x kq keys into the
+// output. The rows of one query are then read by split CUs instead of one (at 1M x 384, batch 8:
+// about 1,000 FP8 candidates a query, 1.5 MB of rows, on one CU each; 32 shares: FP8 50.7k ->
+// 59.9k QPS). The merge is a launch of its own: a last-block hand-off inside this kernel needs
+// device-scope fences, which write back and invalidate the L2 caches of the XCDs (the kernel
+// took 99 instead of 21 us at batch 128).
+template <int D16, bool L2, int CAP, int NT>
+__global__ __launch_bounds__(NT) void rescore_share_kernel(
+    const float* __restrict__ base, uint32_t dp, uint32_t d, const float* __restrict__ queries,
+    const uint32_t* __restrict__ ks, uint32_t k, const float* __restrict__ E,
+    const uint64_t* __restrict__ cand, const uint32_t* __restrict__ count, uint32_t ldo,
+    uint32_t* __restrict__ overflow_flags, uint32_t* __restrict__ n_overflow,
+    uint32_t* __restrict__ rescored, uint32_t cmin, int final_pass, const uint32_t* __restrict__ qmap,
+    uint32_t split, uint64_t* __restrict__ rpart) {
+  extern __shared__ uint64_t dyn[];
+  uint64_t* buf = dyn;                                // CAP keys
+  float* q_lds = reinterpret_cast<float*>(dyn + CAP);  // dp floats
+  __shared__ uint32_t nsel, nall, ntop;
+  __shared__ uint32_t hist[256], sel[2];
+  __shared__ uint64_t top[kTopCap];
+  // block b: share b % split of query b / split (qmap: of query qmap[b / split])
+  const uint32_t bq = blockIdx.x / split, share = blockIdx.x % split;
+  const uint32_t q = qmap ? qmap[bq] : bq;
+  if (q == kNoneId) return;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, t = lane & 3, g = lane >> 2;
+  const uint32_t kq = min(ks ? ks[q] : k, ldo);
+  const uint32_t c = count[q];
+  if (c < cmin) return;  // done by the previous launch
+  if (c > uint32_t(CAP)) {
+    if (final_pass && tid == 0 && share == 0) {
+      overflow_flags[q] = 1;
+      rescored[q] = 0;
+      atomicAdd(n_overflow, 1u);
+    }
+    return;
+  }
+  if (tid == 0 && share == 0) overflow_flags[q] = 0;
+  for (uint32_t i = tid; i < dp; i += NT) q_lds[i] = i < d ? queries[size_t(q) * d + i] : 0.f;
+  // Dk = the kk-th smallest estimate; keep dh <= Dk + 2E (compacted, in no particular order; a
+  // share keeps its own candidates and counts them all)
+  const uint64_t* cq = cand + size_t(q) * kCandCap;
+  for (uint32_t i = tid; i < c; i += NT) buf[i] = cq[i];
+  if (tid == 0) nsel = nall = 0;
+  __syncthreads();
+  const uint32_t kk = min(kq, c);
+  if (kk > 0) {
+    const float Dk = unord_bits(block_select_hi32<NT>(buf, c, kk, hist, sel, tid));
+    const float lim = round_up_f(double(Dk) + 2.0 * double(E[q]) * (1.0 + 1e-6) + 0x1p-20);
+    __syncthreads();  // every read of buf by the select is done
+    for (uint32_t i = tid; i < c; i += NT) {
+      const uint64_t v = cq[i];
+      if (key_dist(v) <= lim) {
+        atomicAdd(&nall, 1u);
+        if (i % split == share) buf[atomicAdd(&nsel, 1u)] = v;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t r = nsel;
+  if (tid == 0 && share == 0) rescored[q] = nall;
+  // exact distances, 16 rows per wave instruction group, NT / 64 waves; a quad reads its
+  // candidate's row id and later overwrites that slot with the exact key
+  for (uint32_t base_i = (tid >> 6) * 16; base_i < r; base_i += 16 * (NT / 64)) {
+    const uint32_t i = base_i + g;
+    bool ok[1] = {i < r};
+    const uint32_t row = key_id(buf[ok[0] ? i : 0]);
+    const float* xr[1] = {base + size_t(row) * dp};
+    float dist[1][1];
+    quad_dists<1, 1, D16, L2, 16, kLoadPlain>(xr, ok, q_lds, dp, d, int(t), dist);
+    if (ok[0] && t == 0) buf[i] = make_key(dist[0][0], row);
+  }
+  __syncthreads();
+  // this share's first kq keys (kPadKey past its r) to its part
+  uint64_t* rp = rpart + (size_t(q) * split + share) * ldo;
+  rank_first<NT>(buf, r, kq, hist, sel, top, &ntop, tid, [&](uint32_t rank, uint64_t v) { rp[rank] = v; });
+  for (uint32_t i = r + tid; i < kq; i += NT) rp[i] = kPadKey;
+}
+
+// The output of the queries whose rescore was split: query q's split parts (share s at
+// rpart + (q * split + s) * ldo, kq keys each, kPadKey where a share had fewer) ranked together.
+// One 256-thread block per query; queries above the candidate capacity are left to the exact
+// fallback, as by rescore_kernel.
+__global__ __launch_bounds__(256) void rescore_merge_kernel(const uint32_t* __restrict__ ks, uint32_t k,
+                                                            const uint32_t* __restrict__ count, uint32_t split,
+                                                            const uint64_t* __restrict__ rpart,
+                                                            uint32_t* __restrict__ out_ids,
+                                                            float* __restrict__ out_dists, uint32_t ldo) {
+  extern __shared__ uint64_t dyn[];  // the next power of two >= max(512, split * ldo) keys
+  __shared__ uint32_t nsel, ntop;
+  __shared__ uint32_t hist[256], sel[2];
+  __shared__ uint64_t top[kTopCap];
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  if (count[q] > uint32_t(kCandCap)) return;
+  const uint32_t kq = min(ks ? ks[q] : k, ldo);
+  uint32_t* oi = out_ids + size_t(q) * ldo;
+  float* od = out_dists + size_t(q) * ldo;
+  if (tid == 0) nsel = 0;
+  __syncthreads();
+  const uint64_t* parts = rpart + size_t(q) * split * ldo;
+  for (uint32_t i = tid; i < split * kq; i += 256) {
+    const uint32_t s = i / kq;
+    const uint64_t v = parts[size_t(s) * ldo + (i - s * kq)];
+    if (v != kPadKey) dyn[atomicAdd(&nsel, 1u)] = v;
+  }
+  __syncthreads();
+  const uint32_t r = nsel;
+  rank_first<256>(dyn, r, kq, hist, sel, top, &ntop, tid, [&](uint32_t rank, uint64_t v) {
+    oi[rank] = key_id(v);
+    od[rank] = key_dist(v);
+  });
+  for (uint32_t i = min(r, kq) + tid; i < ldo; i += 256) {
     oi[i] = kNoneId;
     od[i] = __builtin_inff();
   }
@@ -1707,12 +1875,14 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* queries, uint32_t nq, const uint32_t* ks, uint32_t k,
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
-                          uint32_t* rescored, uint32_t* sig, uint32_t* qmap) {
+                          uint32_t* rescored, uint32_t* sig, uint32_t* qmap, uint32_t split, uint64_t* rpart) {
   // sig / qmap given: order the queries of a large batch (rescore_order_kernel)
   // every query's flag is set or cleared by the launch that handles it
   const size_t lds_small = size_t(kCandSmall) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const bool wide = nq <= 256;  // fewer queries (blocks) than CUs: 16 waves per query, one launch
+  // shares per query (wide launches only; the caller sizes rpart for them)
+  if (!wide || !rpart || split == 0 || size_t(split) * ldo > size_t(kCandCap)) split = 1;
   // larger batches: queries in min-hash order, XCD by XCD (rescore_order_kernel)
   const uint32_t* order = nullptr;
   uint32_t nb = nq;
@@ -1725,25 +1895,40 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 #define VSG_RESCORE(D16, L2V)                                                                         \
   do {                                                                                                \
     if (wide) { /* one block per query either way: one launch with the full buffer */               \
-      hipError_t w_ = hipFuncSetAttribute(                                                            \
-          reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 1024>),                   \
-          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                      \
+      const void* f_ = split > 1 ? reinterpret_cast<const void*>(&rescore_share_kernel<D16, L2V, kCandCap, 1024>) \
+                                 : reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 1024>); \
+      hipError_t w_ = hipFuncSetAttribute(f_, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));  \
       if (w_ != hipSuccess) return w_;                                                                \
-      hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap, 1024>), dim3(nq), dim3(1024), lds, st,    \
-                         base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,         \
-                         n_overflow, rescored, 0u, 1, nullptr);                                       \
+      if (split == 1)                                                                                 \
+        hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap, 1024>), dim3(nq), dim3(1024), lds, st, \
+                           base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,       \
+                           n_overflow, rescored, 0u, 1, nullptr);                                     \
+      else                                                                                            \
+        hipLaunchKernelGGL((rescore_share_kernel<D16, L2V, kCandCap, 1024>), dim3(nq * split),        \
+                           dim3(1024), lds, st, base, dp, d, queries, ks, k, E, cand, count, ldo,     \
+                           flags, n_overflow, rescored, 0u, 1, nullptr, split, rpart);                \
+      if (split > 1) { /* split * ldo <= kCandCap keys; the sort of rank_first pads to 2^j >= 512 */ \
+        size_t m2_ = 512;                                                                             \
+        while (m2_ < size_t(split) * ldo) m2_ <<= 1;                                                  \
+        const size_t lds_m = m2_ * sizeof(uint64_t);                                                  \
+        hipError_t m_ = hipFuncSetAttribute(reinterpret_cast<const void*>(&rescore_merge_kernel),      \
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_m));   \
+        if (m_ != hipSuccess) return m_;                                                              \
+        hipLaunchKernelGGL(rescore_merge_kernel, dim3(nq), dim3(256), lds_m, st, ks, k, count, split,  \
+                           rpart, ids, dists, ldo);                                                   \
+      }                                                                                               \
       break;                                                                                          \
     }                                                                                                 \
     hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandSmall, 256>), dim3(nb), dim3(256), lds_small,    \
                        st, base, dp, d, queries, ks, k, E, cand, count, ids, dists, ldo, flags,       \
-                       n_overflow, rescored, 0u, 0, order);                                           \
+                       n_overflow, rescored, 0u, 0, order);                     \
     hipError_t e_ = hipFuncSetAttribute(                                                              \
         reinterpret_cast<const void*>(&rescore_kernel<D16, L2V, kCandCap, 256>),                      \
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                                        \
     if (e_ != hipSuccess) return e_;                                                                  \
     hipLaunchKernelGGL((rescore_kernel<D16, L2V, kCandCap, 256>), dim3(nb), dim3(256), lds, st, base, dp, \
                        d, queries, ks, k, E, cand, count, ids, dists, ldo, flags, n_overflow,          \
-                       rescored, uint32_t(kCandSmall) + 1, 1, order);                                 \
+                       rescored, uint32_t(kCandSmall) + 1, 1, order);           \
   } while (0)
   if (l2) {
     if (d == 384) VSG_RESCORE(24, true);
@@ -1761,6 +1946,19 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 }
 
 int short_cand_cap() { return kCandCap; }
+
+// Blocks per query of the rescore of a small batch: about one block per CU in all, at most 32
+// shares, and the shares' first-k keys within one candidate buffer for the final rank; none below
+// 4 shares, where the merge launch costs what the shares save (at 1M x 384: FP8 batch 8 and 32
+// +15 %, batch 128 with 2 shares no faster).
+uint32_t short_rescore_split(uint32_t nq, uint32_t k, int num_cus) {
+  if (nq == 0 || nq > 256 || num_cus <= 0) return 1;
+  uint32_t s = std::max(1u, uint32_t(num_cus) / nq);
+  if (s < 4) return 1;
+  s = std::min(s, 32u);
+  s = std::min(s, std::max(1u, uint32_t(kCandCap) / std::max(1u, k)));
+  return s;
+}
 
 
 }  // namespace vsg
