@@ -313,8 +313,9 @@ def test_shortlist_matches_oracle(mixture_70k, method, k):
 @pytest.mark.parametrize("k", [10, 100])
 def test_shortlist_small_batches(mixture_70k, method, nq, k):
     # up to 128 queries (d <= 384) take the queries-stationary pass 2: same bits as the oracle,
-    # and as the same queries inside a larger batch (rows-stationary pass 2); up to 64 queries
-    # also rescore each query in several blocks (64, 16, 4: 4, 16, 32 shares) and merge them
+    # and as the same queries inside a larger batch (rows-stationary pass 2); up to 128 queries
+    # also rescore each query in several blocks (128, 100, 64, 16, 4: 2, 2, 4, 16, 32 shares)
+    # and merge them
     x, q, gpu = mixture_70k
     got = gpu.search(q[:nq], k, method=method)
     assert_same(got, oracle.search(x, q[:nq], k, "ip", threads=8))

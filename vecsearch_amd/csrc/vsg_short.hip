@@ -1424,7 +1424,7 @@ __global__ __launch_bounds__(NT) void rescore_kernel(
   }
 }
 
-// The rescore of a batch of at most a quarter as many queries as CUs, with `split` blocks a
+// The rescore of a batch of at most half as many queries as CUs, with `split` blocks a
 // query: block s rescores the candidates i with i % split == s (after the same select and cut as
 // rescore_kernel, over all of the query's candidates) and writes its first kq exact keys to
 // rpart (kPadKey past its own); rescore_merge_kernel then ranks the split x kq keys into the
@@ -1948,13 +1948,11 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
 int short_cand_cap() { return kCandCap; }
 
 // Blocks per query of the rescore of a small batch: about one block per CU in all, at most 32
-// shares, and the shares' first-k keys within one candidate buffer for the final rank; none below
-// 4 shares, where the merge launch costs what the shares save (at 1M x 384: FP8 batch 8 and 32
-// +15 %, batch 128 with 2 shares no faster).
+// shares, and the shares' first-k keys within one candidate buffer for the final rank (at
+// 1M x 384: FP8 batch 8 and 32 +15 %, batch 128 with 2 shares +4 %).
 uint32_t short_rescore_split(uint32_t nq, uint32_t k, int num_cus) {
   if (nq == 0 || nq > 256 || num_cus <= 0) return 1;
   uint32_t s = std::max(1u, uint32_t(num_cus) / nq);
-  if (s < 4) return 1;
   s = std::min(s, 32u);
   s = std::min(s, std::max(1u, uint32_t(kCandCap) / std::max(1u, k)));
   return s;
