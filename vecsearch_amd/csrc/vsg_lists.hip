@@ -139,21 +139,18 @@ __global__ __launch_bounds__(kBlock) void lists_scan_kernel(
 // the item) and enter the FMAs as scalar operands: no LDS traffic for them and no cross-lane
 // reduction. They are read in item order from the copy lists_gather_queries_kernel makes.
 //
-// Rows reach the lanes through LDS: a wave loads each pair of 16-dim steps of its 64 rows with
-// eight LDS-DMA loads, eight lanes per row, so that every request is a whole 128-byte line of one
-// row (a load's cost follows the number of rows it touches more than its bytes: at 64 bytes a row
-// and 16 rows a load the filtered batch took 1.16 ms, 0.84 ms with every load on one row), one
-// pair ahead in a two-slot ring, and each lane then reads its own row's 64 bytes of a step back.
-// A row's eight 16-byte chunks are stored XOR-swizzled by row & 7, so the 8 lanes of a read
-// group touch 8 different bank groups.
+// Rows reach the lanes through LDS: a wave loads each 16-dim step of its 64 rows with four
+// LDS-DMA loads, four lanes per row (16 cache lines per instruction instead of 64), two steps
+// ahead in a three-slot ring, and each lane then reads its own row's 64 bytes back. A row's four
+// 16-byte chunks are stored XOR-swizzled by (row >> 2) & 3, so the 16 lanes of a read touch 16
+// different bank groups.
 //
-// The round's distances (512 rows x 8 queries, as order-preserving bits) go to LDS; then wave b
-// alone offers query b's keys to the query's k-list, which lives in its registers for the whole
-// item (offer64: one sorted merge for many admitted keys); the keys' row ids come from registers
-// that the wave loaded at the round's start. One list per query sees every row of the chunk, so
-// the inserts are those of one sequential scan. At the end of the item wave b writes the list as
-// the query's chunk list and lowers the query's device-wide threshold, as the kernels above do;
-// a workgroup runs its items one after another, so its later chunks start from it.
+// The round's keys (512 rows x 8 queries) go to LDS; then wave b alone offers query b's keys to
+// the query's k-list, which lives in its registers for the whole item (offer64: one sorted merge
+// for many admitted keys). One list per query sees every row of the chunk, so the inserts are
+// those of one sequential scan. At the end of the item wave b writes the list as the query's
+// chunk list and lowers the query's device-wide threshold, as the kernels above do; a workgroup
+// runs its items one after another, so its later chunks start from it.
 //
 // Items come from eight queues, one per XCD. Workgroups are dealt to the XCDs in turn, so
 // workgroup b starts on queue b % 8, where the planner put every query block of a chunk; a
@@ -165,17 +162,17 @@ constexpr uint32_t kQueues = 8;  // one per XCD
 constexpr uint32_t kCounterWords = 32;  // [0] the single queue, [1..8] the XCD queues,
 constexpr uint32_t kBoundsAt = 16;      // [16..24] the XCD queues' item ranges
 constexpr uint32_t kLaneRound = 64 * kLaneWaves;                    // rows per round
-constexpr size_t kLaneKeyBytes = size_t(kLaneQ) * kLaneRound * 4;  // 16 KB of distance bits
-constexpr size_t kLaneStage = 8192;                                // two 16-dim steps of 64 rows
-constexpr int kLaneSlots = 2;                                     // ring slots per wave
-// two distance buffers (rounds alternate; the item index is passed in the second, which no
-// round writes before every wave has read it) and the waves' rings: 160 KB, a CU's whole LDS
+constexpr size_t kLaneKeyBytes = size_t(kLaneQ) * kLaneRound * 8;  // 32 KB
+constexpr size_t kLaneStage = 4096;                                // one step of 64 rows
+constexpr int kLaneSlots = 3;                                     // ring slots per wave
+// two key buffers (rounds alternate; the item index is passed in the second, which no round
+// writes before every wave has read it) and the waves' rings: 160 KB, a CU's whole LDS
 constexpr size_t kLaneLds = 2 * kLaneKeyBytes + size_t(kLaneWaves) * kLaneSlots * kLaneStage;
 static_assert(kLaneLds <= 160 * 1024, "one workgroup a CU");
 
-// Byte offset of 16-byte chunk j (0..7) of row r (0..63) in a staged step pair.
+// Byte offset of 16-byte chunk j of row r (0..63) in a staged step.
 __device__ __forceinline__ uint32_t lane_slot(uint32_t r, uint32_t j) {
-  return (r >> 3) * 1024 + (r & 7) * 128 + ((j ^ (r & 7)) << 4);
+  return (r >> 4) * 1024 + (r & 15) * 64 + ((j ^ ((r >> 2) & 3)) << 4);
 }
 
 // One 16-dim step of NQ queries against the lane's row. The query values come in pairs of
@@ -237,48 +234,46 @@ __device__ __forceinline__ void lane_fma_step(const float4 (&xv)[4], const float
   }
 }
 
-// Distance bits of the wave's 64 rows x NQ queries into dist_s[b * kLaneRound + 64 * wave + lane].
-// src[c]: this lane's global address for DMA load c of step pair 0 (row 8c + lane / 8 of the
-// wave's rows, the chunk that lands at this lane's slot); stage: the wave's two 8 KB slots.
+// Keys of the wave's 64 rows x NQ queries into key_s[b * kLaneRound + 64 * wave + lane].
+// src[c]: this lane's global address for DMA load c of step 0 (row 16c + lane / 4 of the
+// wave's rows, the chunk that lands at this lane's slot); stage: the wave's three 4 KB slots.
 template <int NQ, int D16, bool L2>
-__device__ __forceinline__ void lane_rows(const float* const (&src)[8], uint8_t* stage, const float* __restrict__ qb,
-                                          uint32_t d, uint32_t nd16, uint32_t* dist_s, int wave, int lane) {
+__device__ __forceinline__ void lane_rows(const float* const (&src)[4], uint8_t* stage, const float* __restrict__ qb,
+                                          uint32_t d, uint32_t nd16, bool ok, uint32_t row, uint64_t* key_s,
+                                          int wave, int lane) {
   float acc[NQ][16];
 #pragma unroll
   for (int b = 0; b < NQ; ++b)
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[b][j] = 0.f;
-  const uint32_t last = nd16 - 1, lastp = nd16 / 2 - 1;  // nd16 is even (d a multiple of 32)
-  auto issue = [&](uint32_t slot, uint32_t p) __attribute__((always_inline)) {
+  const uint32_t last = nd16 - 1;
+  auto issue = [&](uint32_t slot, uint32_t s) __attribute__((always_inline)) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) load_lds16(src[c] + 32 * p, stage + slot * kLaneStage + c * 1024);
+    for (int c = 0; c < 4; ++c) load_lds16(src[c] + 16 * s, stage + slot * kLaneStage + c * 1024);
   };
   issue(0, 0);
+  issue(1, last >= 1 ? 1 : last);
+  uint32_t slot = 0;  // slot of step s; step s + 2 goes to the slot read at step s - 1
   float qv[2][32];  // query values in scalar registers (lane_fma_step)
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     qv[0][i] = qb[i];
     qv[0][16 + i] = qb[size_t(d) + i];
   }
-  for (uint32_t p = 0; p <= lastp; ++p) {
-    // the slot read by pair p - 1 takes pair p + 1 (past the end: pair p again, never read)
-    issue((p + 1) & 1, p < lastp ? p + 1 : p);
-    wait_vm<8>();  // pair p has landed
-    const uint8_t* cur = stage + (p & 1) * kLaneStage;
+  for (uint32_t s = 0; s < nd16; ++s) {
+    issue(slot == 0 ? 2 : slot - 1, s + 2 <= last ? s + 2 : last);  // past the end: unread slots
+    wait_vm<8>();                                                   // step s has landed
+    const uint8_t* cur = stage + slot * kLaneStage;
+    slot = slot == 2 ? 0 : slot + 1;
+    float4 xv[4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t s = 2 * p + h;
-      float4 xv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        xv[j] = *reinterpret_cast<const float4*>(cur + lane_slot(uint32_t(lane), uint32_t(4 * h + j)));
-      // the query row stride as a constant where the width is one (the loads' offsets are then
-      // immediates rather than scalar-register arithmetic)
-      lane_fma_step<NQ, L2>(xv, qb + 16 * s, qb + 16 * (s < last ? s + 1 : s),
-                            D16 > 0 ? size_t(D16) * 16 : size_t(d), acc, qv);
-    }
+    for (int j = 0; j < 4; ++j) xv[j] = *reinterpret_cast<const float4*>(cur + lane_slot(uint32_t(lane), uint32_t(j)));
+    // the query row stride as a constant where the width is one (the loads' offsets are then
+    // immediates rather than scalar-register arithmetic)
+    lane_fma_step<NQ, L2>(xv, qb + 16 * s, qb + 16 * (s < last ? s + 1 : s), D16 > 0 ? size_t(D16) * 16 : size_t(d),
+                          acc, qv);
   }
-  wait_vm<0>();  // the extra loads past the last pair land before the slots are reused
+  wait_vm<0>();  // the extra loads past the last step land before the slots are reused
   // the tree of hsum256 on (acc0 + acc1): v_j = a_j + a_{j+8}, w_c = v_c + v_{c+4},
   // r = (w0 + w1) + (w2 + w3)
 #pragma unroll
@@ -288,7 +283,7 @@ __device__ __forceinline__ void lane_rows(const float* const (&src)[8], uint8_t*
     for (int j = 0; j < 8; ++j) v[j] = acc[b][j] + acc[b][j + 8];
     const float w0 = v[0] + v[4], w1 = v[1] + v[5], w2 = v[2] + v[6], w3 = v[3] + v[7];
     const float r = (w0 + w1) + (w2 + w3);
-    dist_s[b * kLaneRound + 64 * wave + lane] = ord_bits(L2 ? r : 1.0f - r);
+    key_s[b * kLaneRound + 64 * wave + lane] = ok ? make_key(L2 ? r : 1.0f - r, row) : kPadKey;
   }
 }
 
@@ -300,10 +295,10 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
     uint64_t* __restrict__ part, uint32_t* __restrict__ counter, uint64_t* __restrict__ qthr) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint32_t* dist2 = reinterpret_cast<uint32_t*>(lds_raw);  // 2 x kLaneQ x kLaneRound
+  uint64_t* keys2 = reinterpret_cast<uint64_t*>(lds_raw);  // 2 x kLaneQ x kLaneRound
   uint8_t* stage = lds_raw + 2 * kLaneKeyBytes + size_t(wave) * kLaneSlots * kLaneStage;  // this wave's ring
-  // the item index: in the second distance buffer, first written by the item's second round,
-  // after the first round's barrier (every wave has read the index by then)
+  // the item index: in the second key buffer, first written by the item's second round, after
+  // the first round's barrier (every wave has read the index by then)
   uint32_t* item_s = reinterpret_cast<uint32_t*>(lds_raw + kLaneKeyBytes);
   const uint32_t nd16 = D16 > 0 ? uint32_t(D16) : d >> 4;
   const uint32_t* bounds = counter + kBoundsAt;
@@ -340,63 +335,47 @@ __global__ __launch_bounds__(64 * kLaneWaves, 1) void lists_lane_kernel(
     WaveList<KP> lst;
     lst.init();
 
-    uint32_t par = 0;  // distance buffer of the round
+    uint32_t par = 0;  // key buffer of the round
     for (uint32_t r0 = w.lb; r0 < w.le; r0 += kLaneRound, par ^= 1u) {
       // Round r writes buffer r % 2 and its owners then offer from it. Round r + 1 writes the
       // other buffer, last read by round r - 1's offers, which every wave finished before round
       // r's barrier: one barrier a round, and a wave with few keys to offer goes on to the next
       // round's rows while the others still offer.
-      uint32_t* dist_s = dist2 + size_t(par) * (kLaneKeyBytes / 4);
-      const uint32_t nw = min(uint32_t(kLaneWaves), (w.le - r0 + 63) / 64);  // waves with rows
-      // an owner's row ids of the round's 512 rows (column c: rows r0 + 64c ..), loaded now so
-      // that they have landed when its offers need them
-      uint32_t idv[kLaneWaves];
-#pragma unroll
-      for (int c = 0; c < kLaneWaves; ++c) {
-        const uint32_t pos = r0 + 64u * uint32_t(c) + uint32_t(lane);
-        idv[c] = owner && kq > 0 && !w.direct && uint32_t(c) < nw ? allow_ids[pos < w.le ? pos : w.lb] : pos;
-      }
+      uint64_t* key_s = keys2 + size_t(par) * (kLaneKeyBytes / 8);
       const uint32_t my0 = r0 + 64u * uint32_t(wave);
+      const uint32_t idx = my0 + uint32_t(lane);
+      const bool ok = idx < w.le;
       if (my0 < w.le) {
-        const uint32_t idx = my0 + uint32_t(lane);
-        const bool ok = idx < w.le;
         const uint32_t row = w.direct ? (ok ? idx : w.lb) : allow_ids[ok ? idx : w.lb];
-        const float* src[8];
+        const float* src[4];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const uint32_t rr = 8u * c + (uint32_t(lane) >> 3);  // row of the wave that DMA load c fills here
-          const uint32_t jj = (uint32_t(lane) & 7) ^ (rr & 7);  // its chunk at this slot
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t rr = 16u * c + (uint32_t(lane) >> 2);  // row of the wave that DMA load c fills here
+          const uint32_t jj = (uint32_t(lane) & 3) ^ ((uint32_t(lane) >> 4) & 3);  // its chunk at this slot
           const uint32_t rid = uint32_t(__shfl(int(row), int(rr), 64));
           src[c] = base + size_t(rid) * dp + 4 * jj;
         }
         // the item's queries, rounded up to a pair
         if (nq <= 2) {
-          lane_rows<2, D16, L2>(src, stage, qb, d, nd16, dist_s, wave, lane);
+          lane_rows<2, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
         } else if (kLaneQ <= 4 || nq <= 4) {
-          lane_rows<4, D16, L2>(src, stage, qb, d, nd16, dist_s, wave, lane);
+          lane_rows<4, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
         } else if constexpr (kLaneQ > 4) {
-          if (nq <= 6) lane_rows<6, D16, L2>(src, stage, qb, d, nd16, dist_s, wave, lane);
-          else lane_rows<8, D16, L2>(src, stage, qb, d, nd16, dist_s, wave, lane);
+          if (nq <= 6) lane_rows<6, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
+          else lane_rows<8, D16, L2>(src, stage, qb, d, nd16, ok, row, key_s, wave, lane);
         }
       }
-      __syncthreads();  // the round's distances are in LDS
+      __syncthreads();  // the round's keys are in LDS
       if (owner && kq > 0) {
         // a tighter bound that the query's other chunks published since (a relaxed load of a
         // value that only decreases: any value read is a valid bound)
         const uint64_t pub = __hip_atomic_load(qthr + myq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t thr0 = pub < thr ? pub : thr;
         thr = thr0;
-        const uint32_t* dq = dist_s + uint32_t(wave) * kLaneRound;
+        const uint32_t nw = min(uint32_t(kLaneWaves), (w.le - r0 + 63) / 64);  // waves with rows
 #pragma unroll 1
         for (uint32_t c = 0; c < nw; ++c) {
-          const uint32_t pos = r0 + 64 * c + uint32_t(lane);
-          const uint32_t db = dq[64 * c + uint32_t(lane)];
-          // a key below thr has its distance bits at most thr's (rows past the chunk: none)
-          if (ballot(pos < w.le && db <= uint32_t(thr >> 32)) == 0) continue;
-          uint32_t id = idv[0];
-#pragma unroll
-          for (int j = 1; j < kLaneWaves; ++j) id = c == uint32_t(j) ? idv[j] : id;
-          const uint64_t key = pos < w.le ? (uint64_t(db) << 32) | id : kPadKey;
+          const uint64_t key = key_s[uint32_t(wave) * kLaneRound + 64 * c + uint32_t(lane)];
           if (ballot(key < thr) == 0) continue;
           thr = offer64<KP>(lst, thr, key, lane, kq);
         }
