@@ -980,6 +980,8 @@ struct RegionGeometry {
 //           overflowed sets *overflow (the round is then answered by the exact scan);
 //   offsets per query (one wave), exclusive scan over the slices in place, total -> count[q];
 //   place   per slice, each entry at cand[q * cap + offset + its LDS rank] if below cap.
+// A batch of at most 128 queries skips the offsets launch: each place block sums the counts of
+// the slices before its own.
 // The order of a query's candidates is not fixed; the rescore's result does not depend on it.
 // A slice's block has 16 waves, each walking its own regions, so that a CU keeps 16 regions'
 // loads in flight: each wave's loop is a chain of a load and an LDS atomic.
@@ -1046,15 +1048,45 @@ __global__ __launch_bounds__(256) void scatter_offsets_kernel(uint32_t* __restri
   if (lane == 63) count[q] = incl;
 }
 
+// With own_offsets set (a batch of at most kPlaceOwnQ queries), slice_off holds the raw counts
+// of scatter_count_kernel and each slice's block sums the counts of the slices before its own
+// (8 lanes a query), and the last slice's block writes the totals to count: no offsets launch.
+constexpr uint32_t kPlaceOwnQ = kScatterThreads / 8;
 __global__ __launch_bounds__(kScatterThreads) void scatter_place_kernel(
     const uint64_t* __restrict__ emit, const uint32_t* __restrict__ emit_count, uint32_t wave_cap,
     RegionGeometry geo, uint32_t n_regions, uint32_t per, uint32_t nq, uint32_t S,
-    const uint32_t* __restrict__ slice_off, uint64_t* __restrict__ cand, uint32_t cap) {
+    const uint32_t* __restrict__ slice_off, uint64_t* __restrict__ cand, uint32_t cap, int own_offsets,
+    uint32_t* __restrict__ count) {
   extern __shared__ uint32_t base[];  // nq
   __shared__ uint32_t rc[64];
   const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t r0 = s * per, r1 = min(n_regions, r0 + per);
-  for (uint32_t q = tid; q < nq; q += kScatterThreads) base[q] = slice_off[size_t(q) * kScatterSlices + s];
+  if (own_offsets) {
+    const uint32_t q = tid >> 3, part = tid & 7;  // nq <= kPlaceOwnQ: every query has its 8 lanes
+    uint32_t sum = 0;
+    if (q < nq) {
+      const uint32_t* row = slice_off + size_t(q) * kScatterSlices;
+      for (uint32_t j0 = 0; j0 < s; j0 += 128) {  // 16 loads in flight a lane
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const uint32_t j = j0 + part + 8 * u;
+          v[u] = j < s ? row[j] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) sum += v[u];
+      }
+    }
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    sum += __shfl_xor(sum, 4, 64);
+    if (q < nq && part == 0) {
+      base[q] = sum;
+      if (s == S - 1) count[q] = sum + slice_off[size_t(q) * kScatterSlices + s];
+    }
+  } else {
+    for (uint32_t q = tid; q < nq; q += kScatterThreads) base[q] = slice_off[size_t(q) * kScatterSlices + s];
+  }
   for (uint32_t rb = r0; rb < r1; rb += 64) {
     __syncthreads();
     if (tid < min(64u, r1 - rb)) rc[tid] = emit_count[rb + tid];
@@ -1710,10 +1742,13 @@ hipError_t launch_scatter(int prec, const ShortScanArgs& a) {
   const size_t lds = size_t(a.nq) * 4;
   hipLaunchKernelGGL(scatter_count_kernel, dim3(S), dim3(kScatterThreads), lds, a.st, a.emit, a.emit_count,
                      a.wave_cap, geo, n_regions, per, a.nq, S, a.slice, a.overflow);
-  hipLaunchKernelGGL(scatter_offsets_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, a.st, a.slice, S, a.nq,
-                     a.count);
+  // a small batch's place blocks sum their own offsets (one launch less, about 5 us)
+  const int own = a.nq <= kPlaceOwnQ ? 1 : 0;
+  if (!own)
+    hipLaunchKernelGGL(scatter_offsets_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, a.st, a.slice, S, a.nq,
+                       a.count);
   hipLaunchKernelGGL(scatter_place_kernel, dim3(S), dim3(kScatterThreads), lds, a.st, a.emit, a.emit_count,
-                     a.wave_cap, geo, n_regions, per, a.nq, S, a.slice, a.cand, a.cap);
+                     a.wave_cap, geo, n_regions, per, a.nq, S, a.slice, a.cand, a.cap, own, a.count);
   return hipGetLastError();
 }
 
