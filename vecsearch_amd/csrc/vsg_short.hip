@@ -831,7 +831,9 @@ __global__ __launch_bounds__(64 * kQsWaves, qs_bpc(PREC, KS)) void mfma_qs_kerne
 // same regions and the same entries as that kernel. The products are summed in another order
 // than in the other passes, so an estimate may differ from theirs in its last bits; each is
 // within the bound E (any fp32 order of the d products' sum is within 2 gamma_d |q^| |x^|).
-constexpr int kQkDepth = 6;
+// fragments in flight a wave: pass 2 at 1M x 384, batch 8 / 128: depth 3 0.118 / 0.130 ms,
+// 4 0.118 / 0.127, 5 0.119 / 0.128, 6 0.121 / 0.130, 8 0.164 / 0.180 (registers spill)
+constexpr int kQkDepth = 4;
 // The queries-stationary passes stream their rows with non-temporal loads when one query group
 // reads them once (at 1M x 384, batch 128: BF16 0.151 -> 0.127 ms, FP8 0.086 -> 0.071 ms); with
 // two to four groups the later groups re-read the rows from L2 and plain loads are as fast or
