@@ -61,8 +61,10 @@ def p2():
 
 
 @pytest.mark.parametrize("method", ["naive", "skinny"])
-@pytest.mark.parametrize("batch", [1, 8, 128, 4096])
+@pytest.mark.parametrize("batch", [1, 8, 48, 128, 4096])
 def test_p2_exact_scan(p2, method, batch):
+    # skinny from 32 queries runs the filtered scan's block-shared (32-63) or one-lane-per-row
+    # (64 and up) tile over every row; naive and smaller batches the streaming scan
     ids, d = p2["idx"].search(p2["q"][:batch], 10, method)
     rows = SAMPLE[SAMPLE < batch]
     check_rows(ids, d, rows, p2["truth"][0][:len(rows)], p2["truth"][1][:len(rows)], f"P2 {method} b={batch}")

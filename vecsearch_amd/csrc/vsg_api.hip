@@ -588,13 +588,14 @@ int enqueue_exact_in(vsg_index* h, uint64_t* part, size_t part_keys, const float
 vsg::ListsTiles lists_tiles_for(vsg_index* h, size_t nq, size_t kmax, const uint64_t* range);
 
 // Unmasked exact batches of at least kLaneExactBatch queries run as a filtered batch whose every
-// query is unfiltered, on the one-lane-per-row tile (unless a filtered-scan tile is fixed or
-// tuned for the class): its work items are chunk-major, so a chunk's rows are read from HBM once
-// and from L2 by the chunk's other query blocks, where the streaming scan reads the whole base
-// once per 8 queries (at 1M x 384: 64 queries 2.32 -> 1.47 ms, 256 queries 8.98 -> 4.86 ms, the
-// same bits). Planning uploads the plan and synchronises the stream, so a captured step keeps
-// the streaming scan.
-constexpr size_t kLaneExactBatch = 64;
+// query is unfiltered, on the one-lane-per-row tile from 64 queries and the block-shared tile
+// (32 queries an item) below, unless a filtered-scan tile is fixed or tuned for the class: the
+// work items are chunk-major, so a chunk's rows are read from HBM once and from L2 by the
+// chunk's other query blocks, where the streaming scan reads the whole base once per 8 queries
+// (at 1M x 384: 32 queries 1.18 -> 0.78 ms, 64 queries 2.32 -> 1.47 ms, 256 queries 8.98 ->
+// 4.86 ms, the same bits). Planning uploads the plan and synchronises the stream, so a captured
+// step keeps the streaming scan.
+constexpr size_t kLaneExactBatch = 32;
 
 int enqueue_exact_lanes(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, uint32_t* d_ids,
                         float* d_dists, hipStream_t st) {
@@ -604,9 +605,16 @@ int enqueue_exact_lanes(vsg_index* h, Workspace& w, const float* d_q, size_t nq,
     const std::vector<uint32_t> ks(rq, uint32_t(k));
     vsg::ListsTiles t = lists_tiles_for(h, rq, k, range.data());
     if (t.qb == 0 && t.r == 0 && t.chunk == 0 && t.blocks_per_cu == 0 && t.lanes_per_row == 0) {
-      t.qb = 8;
-      t.r = 1;
-      t.lanes_per_row = 1;
+      if (rq < 64 && vsg::lists_tile_supported(32, 8, 8, h->dim)) {
+        t.qb = 32;
+        t.r = 8;
+        t.chunk = 2048;
+        t.lanes_per_row = 8;
+      } else {
+        t.qb = 8;
+        t.r = 1;
+        t.lanes_per_row = 1;
+      }
     }
     vsg::ListsLaunch L;
     const int rc = vsg::prepare_lists_device(w.lists, h->n, h->dim, h->dp, h->metric, h->num_cus, st, rq, ks.data(),
