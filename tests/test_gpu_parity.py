@@ -132,6 +132,22 @@ def test_mask_filter(frac, metric):
         gpu.search(q, 10, filter=np.ones(5, bool))
 
 
+@pytest.mark.parametrize("frac", [0.9, 0.5, 0.3])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_mask_filter_large_batch(frac, metric):
+    # a dense filter over 256 queries and more runs the filtered scan with one shared id list
+    # (the tile autotuner's first call included): the oracle's bits, as a small batch gets them
+    x, q = data(70000, 64, 30, metric == "ip"), data(300, 64, 31, metric == "ip")
+    mask = np.random.default_rng(32).random(len(x)) < frac
+    gpu = vs.GpuBruteForceIndex(64, metric)
+    gpu.add(x)
+    exp = oracle.search(x, q, 10, metric, threads=8, mask=mask)
+    for _ in range(2):
+        assert_same(gpu.search(q, 10, filter=mask), exp)
+    small = gpu.search(q[:8], 10, filter=mask)
+    np.testing.assert_array_equal(small[0], exp[0][:8])
+
+
 def test_filter_fewer_allowed_than_k():
     x, q = data(5000, 32, 13), data(3, 32, 14)
     mask = np.zeros(len(x), bool)
