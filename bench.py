@@ -80,6 +80,7 @@ def parse():
     ap.add_argument("--serve-dim", type=int, default=768)
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="offered-load time per rate")
     ap.add_argument("--serve-slots", type=int, default=2, help="worker streams of the serving loop")
+    ap.add_argument("--serve-max-batch", type=int, default=4096, help="queries per batch of the serving loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB megabytes (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings), then exit")
@@ -207,13 +208,15 @@ def serving(args, log):
     # warm-up: builds the BF16 copy and tunes the filtered-scan tiles of the common batch shapes;
     # the measured server then never pauses a batch for tuning (classes it has not seen run the
     # planner's tile)
-    with vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=args.serve_slots, method="auto") as warm_srv:
+    with vs.Server(idx, max_batch_queries=args.serve_max_batch, workspace_bytes=8 << 30, slots=args.serve_slots,
+                   method="auto") as warm_srv:
         for r in [make_request() for _ in range(16)]:
             warm_srv.wait(warm_srv.submit(*r))
         for _ in range(3):
             for t in [warm_srv.submit(*r) for r in [make_request() for _ in range(100)]]:
                 warm_srv.wait(t)
-    srv = vs.Server(idx, max_batch_queries=4096, workspace_bytes=8 << 30, slots=args.serve_slots, method="auto",
+    srv = vs.Server(idx, max_batch_queries=args.serve_max_batch, workspace_bytes=8 << 30, slots=args.serve_slots,
+                    method="auto",
                     tune=False)
 
     def run(reqs, rate):
@@ -292,7 +295,7 @@ def serving(args, log):
                         f"the Zipf(1.1) law with rank shift {datagen.F_SHIFT} and cutoff {datagen.F_CUTOFF}: "
                         f"groups of 12 % .. 0.01 % of the rows)",
             "fp64_max_abs_err": err,
-            "server": {"slots": args.serve_slots, "max_batch_queries": 4096, "workspace_bytes": 8 << 30,
+            "server": {"slots": args.serve_slots, "max_batch_queries": args.serve_max_batch, "workspace_bytes": 8 << 30,
                        "tune": False},
             "saturation_requests_per_s": round(sat_rps, 1), "saturation_qps": round(sat_qps, 1),
             "rates": rows, "admission_waits": st["admission_waits"],
