@@ -92,7 +92,11 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
 
 /* The same search on device-resident buffers (queries nq x dim fp32, out_ids / out_dists
  * nq x k), enqueued on `stream` (a hipStream_t; NULL = the index's own stream). Returns after
- * enqueueing; the outputs are valid once the stream reaches that point. */
+ * enqueueing; the outputs are valid once the stream reaches that point. Two paths wait on the
+ * stream before returning: the shortlist, once per round of up to 4,096 queries, to learn
+ * whether a query overflowed its candidates (those are answered again exactly), and an exact
+ * search of 32 or more unmasked queries, which runs as the filtered scan over every row and
+ * uploads its plan first. */
 int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
                       const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream);
 
