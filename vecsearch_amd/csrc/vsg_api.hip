@@ -869,7 +869,8 @@ int enqueue_short(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_
                                   qr, rq, nullptr, uint32_t(k), w.s_E.p, w.s_cand.p, w.s_count.p, ids,
                                   dists, uint32_t(k), w.s_flags.p, w.s_nover.p, w.s_rescored.p,
                                   p == 2 ? w.s_sig.p : nullptr, p == 2 ? w.s_qmap.p : nullptr,
-                                  vsg::short_rescore_split(rq, uint32_t(k), h->num_cus), w.s_rpart.p));
+                                  vsg::short_rescore_split(rq, uint32_t(k), h->num_cus), w.s_rpart.p,
+                                  w.s_rpart.cap));
       ts.launches++;
     }
     if (dev_fallback) {

@@ -93,10 +93,10 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
                           uint32_t* rescored, uint32_t* sig, uint32_t* qmap, uint32_t split,
-                          uint64_t* rpart);
+                          uint64_t* rpart, size_t rpart_keys);
 // sig: nq, qmap: nq + 7 (sig, qmap null: queries in batch order). split: blocks per query of a
-// batch of at most 256 queries (1 where split * ldo exceeds short_cand_cap() or rpart is null);
-// rpart: nq * split * ldo keys.
+// batch of at most 256 queries (1 where split * ldo exceeds short_cand_cap(), rpart is null or
+// its rpart_keys keys hold fewer than nq * split * ldo).
 uint32_t short_rescore_split(uint32_t nq, uint32_t k, int num_cus);
 
 // vsg_lists.hip: the filtered exact scan over per-query allowed-id lists.

@@ -1912,14 +1912,17 @@ hipError_t launch_rescore(hipStream_t st, bool l2, const float* base, uint32_t d
                           const float* queries, uint32_t nq, const uint32_t* ks, uint32_t k,
                           const float* E, const uint64_t* cand, const uint32_t* count, uint32_t* ids,
                           float* dists, uint32_t ldo, uint32_t* flags, uint32_t* n_overflow,
-                          uint32_t* rescored, uint32_t* sig, uint32_t* qmap, uint32_t split, uint64_t* rpart) {
+                          uint32_t* rescored, uint32_t* sig, uint32_t* qmap, uint32_t split, uint64_t* rpart,
+                          size_t rpart_keys) {
   // sig / qmap given: order the queries of a large batch (rescore_order_kernel)
   // every query's flag is set or cleared by the launch that handles it
   const size_t lds_small = size_t(kCandSmall) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const size_t lds = size_t(kCandCap) * sizeof(uint64_t) + size_t(dp) * sizeof(float);
   const bool wide = nq <= 256;  // fewer queries (blocks) than CUs: 16 waves per query, one launch
   // shares per query (wide launches only; the caller sizes rpart for them)
-  if (!wide || !rpart || split == 0 || size_t(split) * ldo > size_t(kCandCap)) split = 1;
+  if (!wide || !rpart || split == 0 || size_t(split) * ldo > size_t(kCandCap) ||
+      size_t(nq) * split * ldo > rpart_keys)
+    split = 1;
   // larger batches: queries in min-hash order, XCD by XCD (rescore_order_kernel)
   const uint32_t* order = nullptr;
   uint32_t nb = nq;
