@@ -148,6 +148,19 @@ def test_mask_filter_large_batch(frac, metric):
     np.testing.assert_array_equal(small[0], exp[0][:8])
 
 
+@pytest.mark.parametrize("metric", ["l2", "ip", "cosine"])
+@pytest.mark.parametrize("nq", [40, 100])
+def test_exact_batch_on_filtered_scan(metric, nq):
+    # 32 queries and more of the exact methods run as the filtered scan over every row
+    # (block-shared tile below 64 queries, one lane per row above): the oracle's bits
+    x, q = data(70000, 64, 40, metric != "l2"), data(nq, 64, 41, metric != "l2")
+    gpu = vs.GpuBruteForceIndex(64, metric)
+    gpu.add(x)
+    exp = oracle.search(x, q, 10, "l2" if metric == "l2" else "ip", threads=8)
+    assert_same(gpu.search(q, 10, method="skinny"), exp)
+    assert_same(gpu.search(q, 10, method="naive"), exp)
+
+
 def test_filter_fewer_allowed_than_k():
     x, q = data(5000, 32, 13), data(3, 32, 14)
     mask = np.zeros(len(x), bool)
