@@ -19,7 +19,7 @@ PROBE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_probes", "
 
 
 def test_mfma_accuracy_within_the_bound_terms():
-    assert os.path.exists(PROBE), "the probe is not built (vecsearch_amd/build.py builds it)"
+    assert os.path.exists(PROBE), "the probe is not built (tests/gpu_probes/build.py builds it)"
     out = subprocess.run([PROBE], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout.strip().splitlines()[-1])
