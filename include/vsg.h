@@ -40,7 +40,9 @@ typedef struct vsg_index vsg_index;
 /* Search method (gpu_brute_force.h:16-21). Every method returns the same ids and distances
  * (the reference CPU index's); they differ only in how the work is done:
  *   NAIVE   the exact streaming scan, one query per pass over the base;
- *   SKINNY  the exact streaming scan, up to 8 queries per pass;
+ *   SKINNY  the exact streaming scan, up to 8 queries per pass (24 or more unmasked queries
+ *           with dim a multiple of 32, on bases of at least 65,536 rows: the filtered scan over
+ *           every row, which shares each chunk's row reads between the queries);
  *   TILED   the certified shortlist with a BF16 MFMA scan (large batches);
  *   BF16    the same as TILED;
  *   FP8     the certified shortlist with an FP8 E4M3 (per-row scale) MFMA scan;
@@ -95,7 +97,7 @@ int vsg_search(vsg_index* h, const float* queries, size_t nq, size_t k, int meth
  * enqueueing; the outputs are valid once the stream reaches that point. Two paths wait on the
  * stream before returning: the shortlist, once per round of up to 4,096 queries, to learn
  * whether a query overflowed its candidates (those are answered again exactly), and an exact
- * search of 32 or more unmasked queries, which runs as the filtered scan over every row and
+ * search of 24 or more unmasked queries, which runs as the filtered scan over every row and
  * uploads its plan first. */
 int vsg_search_device(vsg_index* h, const float* d_queries, size_t nq, size_t k, int method,
                       const uint8_t* d_mask, uint32_t* d_ids, float* d_dists, void* stream);

@@ -149,9 +149,9 @@ def test_mask_filter_large_batch(frac, metric):
 
 
 @pytest.mark.parametrize("metric", ["l2", "ip", "cosine"])
-@pytest.mark.parametrize("nq", [40, 100])
+@pytest.mark.parametrize("nq", [24, 40, 100])
 def test_exact_batch_on_filtered_scan(metric, nq):
-    # 32 queries and more of the exact methods run as the filtered scan over every row
+    # 24 queries and more of the exact methods run as the filtered scan over every row
     # (block-shared tile below 64 queries, one lane per row above): the oracle's bits
     x, q = data(70000, 64, 40, metric != "l2"), data(nq, 64, 41, metric != "l2")
     gpu = vs.GpuBruteForceIndex(64, metric)

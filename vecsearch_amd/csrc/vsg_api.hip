@@ -592,10 +592,11 @@ vsg::ListsTiles lists_tiles_for(vsg_index* h, size_t nq, size_t kmax, const uint
 // (32 queries an item) below, unless a filtered-scan tile is fixed or tuned for the class: the
 // work items are chunk-major, so a chunk's rows are read from HBM once and from L2 by the
 // chunk's other query blocks, where the streaming scan reads the whole base once per 8 queries
-// (at 1M x 384: 32 queries 1.18 -> 0.78 ms, 64 queries 2.32 -> 1.47 ms, 256 queries 8.98 ->
-// 4.86 ms, the same bits). Planning uploads the plan and synchronises the stream, so a captured
-// step keeps the streaming scan.
-constexpr size_t kLaneExactBatch = 32;
+// (at 1M x 384, host calls: 24 queries 0.89 -> 0.74 ms, 32 queries 1.18 -> 0.78 ms, 64 queries
+// 2.32 -> 1.47 ms, 256 queries 8.98 -> 4.86 ms, the same bits; at 16 queries the two are even).
+// Planning uploads the plan and synchronises the stream, so a captured step keeps the streaming
+// scan.
+constexpr size_t kLaneExactBatch = 24;
 
 int enqueue_exact_lanes(vsg_index* h, Workspace& w, const float* d_q, size_t nq, size_t k, uint32_t* d_ids,
                         float* d_dists, hipStream_t st) {
